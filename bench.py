@@ -1,0 +1,132 @@
+#!/usr/bin/env python3
+"""Throughput of the ISLBM stream-collide step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64|bf16|fp16]
+
+One step is one Launch_CollisionStreaming (bulk-flux probe + fused stream-collide kernel)
+over the whole device-resident domain.  Workload: the production periodic hill,
+128 spanwise x 256 streamwise x 128 wall-normal nodes (4,194,304 nodes), Smagorinsky
+Cs 0.1, seeded perturbed equilibrium start; synthetic state, no I/O.  Prints one JSON
+line with MLUPS, the HBM roofline of the stream-collide kernel (HIP events on its own
+stream) and the FP64 CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = "lattie-boltzmann-method-note_of_paperreview_amd"
+sys.path.insert(0, ROOT)
+
+# algorithmic bytes per lattice update: read + write of 19 populations and the
+# rho, u, v, w stores of every step (evolution.h:657-679)
+BYTES_PER_LUP = {"fp64": 38 * 8 + 4 * 8, "fp32": 38 * 4 + 4 * 4, "bf16": 38 * 2 + 4 * 4, "fp16": 38 * 2 + 4 * 4}
+HBM_PEAK_GBS = 8000.0
+
+GRID = dict(nx=128, ny=256, nz=128)
+PHYSICS = {
+    # Re_h 2800, U_b 0.05, Smagorinsky Cs 0.1 on the bisected (first cell = minSize) mesh
+    "production": dict(tau=0.0, re=2800.0, u_ref=0.05, cs_smag=0.1, mesh_mode=0),
+    # the reference's shipped operating point (tau 0.6833, Re 50)
+    "reference": dict(tau=0.6833, re=50.0, u_ref=0.0, cs_smag=0.0, mesh_mode=0),
+}
+
+
+def traffic_from_profiles(kernel_tag: str, prec: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled
+    by the copy-kernel calibration), committed under profiles/; None if not measured."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}_{prec}_128x256x128.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return float(json.load(open(path))["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(physics: dict, steps: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    pyoracle.build()
+    pyoracle.threads(threads)
+    o = pyoracle.Oracle(pyoracle.default_params(**GRID, **{k: v for k, v in physics.items()}))
+    o.init_equilibrium(u0=(0.0, o.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    o.step(1)  # first touch
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    n = GRID["nx"] * GRID["ny"] * GRID["nz"]
+    o.close()
+    return {"value": round(n * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, {steps} steps of the same 128x256x128 workload ({dt:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
+    ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled"])
+    ap.add_argument("--physics", default="production", choices=sorted(PHYSICS))
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures the one-GPU path only; this build has no multi-GPU decomposition "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    pkg = importlib.import_module(PKG)
+    kern = {"auto": pkg.KERNEL_AUTO, "pernode": pkg.KERNEL_PERNODE, "tiled": pkg.KERNEL_TILED}[args.kernel]
+    phys = PHYSICS[args.physics]
+    p = pkg.default_params(**GRID, **phys, precision=args.precision, kernel=kern)
+    s = pkg.Solver(p)
+    s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    n = GRID["nx"] * GRID["ny"] * GRID["nz"]
+    print(f"[bench] {args.precision} {args.physics} kernel={s.kernel_in_use()} warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+    s.step(args.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    total_ms, collide_ms = s.time_steps(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    r = s.macros()[0]
+    finite = bool((r == r).all())
+    value = n * args.steps / wall / 1e6
+    avg_collide_s = collide_ms / 1e3 / args.steps
+    bpl = BYTES_PER_LUP[args.precision]
+    achieved = bpl * n / avg_collide_s / 1e9
+    ktag = "tiled" if s.kernel_in_use() == pkg.KERNEL_TILED else "pernode"
+    traffic = traffic_from_profiles(ktag, args.precision)
+    out = {
+        "metric": "MLUPS", "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
+                                                          "fp16": "f16-storage/f32"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "periodic hill ISLBM D3Q19 MRT, 128x256x128 (spanwise x streamwise x wall-normal)",
+                   "precision": args.precision, "physics": args.physics, **phys, "kernel": ktag,
+                   "state_finite": finite, "event_ms_per_step": round(total_ms / args.steps, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"k_collide_{ktag}", "bytes_per_lup": bpl,
+                     "avg_launch_us": round(avg_collide_s * 1e6, 2)},
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(phys, args.cpu_steps)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,157 @@
+/*
+ * islbm.h -- C ABI of the MI355X (gfx950) ISLBM periodic-hill solver.
+ *
+ * The solver advances the interpolation-supplemented D3Q19 MRT lattice Boltzmann step of
+ * the reference (7-point Lagrange interpolated streaming on a body-fitted tanh mesh over
+ * the ERCOFTAC hill, flat-wall bounce-back, BFL at the hill, body force driven by a
+ * bulk-flux controller) on one GPU with one device-resident domain.
+ *
+ * Axis convention is the reference's: x = spanwise (LX, nx nodes), y = streamwise (LY,
+ * ny nodes), z = wall-normal (LZ, nz nodes).  Host arrays exchanged through this ABI hold
+ * unique nodes only, as [k][j][i] with i fastest; population arrays are [19][k][j][i].
+ * The periodic duplicate planes and ghost layers of the reference's padded layout
+ * (variables.h:16-19) are not stored: x and y periodicity is resolved by indexing.
+ *
+ * Every function returns ISLBM_OK (0) or a negative ISLBM_ERR_* code and never exits the
+ * process (the reference's CHECK_CUDA calls exit(1), common.h:6-16); the message of the
+ * last failure on the calling thread is returned by islbm_last_error().
+ * One solver is driven from one host thread at a time; distinct solvers are independent.
+ */
+#ifndef ISLBM_H
+#define ISLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISLBM_OK 0
+#define ISLBM_ERR_ARG (-1)
+#define ISLBM_ERR_HIP (-2)
+#define ISLBM_ERR_BUDGET (-3)
+#define ISLBM_ERR_STATE (-4)
+#define ISLBM_ERR_UNSUPPORTED (-5)
+
+/* population storage / arithmetic */
+#define ISLBM_FP64 0 /* FP64 storage and arithmetic (the reference's precision)        */
+#define ISLBM_FP32 1 /* FP32 storage and arithmetic                                    */
+#define ISLBM_BF16 2 /* BF16 storage of (f_i - w_i) * s_i, FP32 arithmetic              */
+#define ISLBM_FP16 3 /* FP16 storage of (f_i - w_i) * s_i, FP32 arithmetic              */
+
+/* wall-normal mesh */
+#define ISLBM_MESH_BISECT 0  /* tanh parameter a bisected so that the first cell == minSize
+                                (GetNonuniParameter, initializationTool.h:16-39)         */
+#define ISLBM_MESH_STRETCH 1 /* a = tanh(stretch / 2)                                   */
+
+/* step kernel */
+#define ISLBM_KERNEL_AUTO 0      /* tiled kernel where supported, else per-node kernel   */
+#define ISLBM_KERNEL_PERNODE 1   /* one thread per node, direct neighbour reads          */
+#define ISLBM_KERNEL_TILED 2     /* column-marching, LDS-staged, sum-factorised          */
+
+/* Run-time form of the reference's compile-time parameter set (variables.h:5-102). */
+typedef struct islbm_params {
+    int32_t nx, ny, nz;    /* unique nodes: NX (spanwise), NY (streamwise), NZ (wall-normal) */
+    double lx, ly, lz;     /* LX, LY, LZ (variables.h:6-8)                               */
+    double cfl;            /* CFL; minSize = dt = (LZ-1)/NZ*CFL (variables.h:22-23,76)    */
+    double lxi;            /* LXi, computational wall-normal extent (variables.h:30)      */
+    double hill_alpha;     /* hill-length scale factor; 1 = reference hill (model.h)      */
+    int32_t flat;          /* 1: flat channel (ChannelFunction, model.h:70-73)            */
+    int32_t mesh_mode;     /* ISLBM_MESH_*                                               */
+    double stretch;        /* tanh stretch for ISLBM_MESH_STRETCH                         */
+    double tau;            /* > 0: relaxation time given, Uref = Re*nu (variables.h:95-99) */
+    double re;             /* Re (variables.h:41)                                        */
+    double u_ref;          /* used when tau <= 0: nu = u_ref/Re, tau = 3 nu/dt + 1/2      */
+    double cs_smag;        /* Smagorinsky constant; 0 = the reference MRT                 */
+    int32_t ndtfrc;        /* forcing-controller cadence in steps (variables.h:48)        */
+    int32_t precision;     /* ISLBM_FP64 / FP32 / BF16 / FP16                             */
+    int32_t kernel;        /* ISLBM_KERNEL_*                                              */
+    int32_t device;        /* HIP device ordinal                                          */
+} islbm_params;
+
+typedef struct islbm_solver islbm_solver;
+
+/* The reference's shipped parameter set (variables.h): 32 x 128 x 64, tau 0.6833, Re 50,
+ * CFL 0.6, LXi 10, NDTFRC 10000, bisected mesh; FP64, automatic kernel, device 0. */
+int islbm_default_params(islbm_params* p);
+
+/* Device bytes a solver with these parameters allocates. */
+int islbm_required_bytes(const islbm_params* p, size_t* bytes);
+
+/* Computes the mesh and interpolation/BFL tables in FP64 (replaces AllocateMemory,
+ * GenerateMesh_X/Y/Z, GetIntrplParameter_X/Y/Xi and BFLInitialization,
+ * memory.h:51-171 and initialization.h:51-360) and allocates the device state.
+ * Fails with ISLBM_ERR_BUDGET when the state exceeds hbm_budget bytes (0 = no limit). */
+int islbm_create(const islbm_params* p, size_t hbm_budget, islbm_solver** out);
+int islbm_destroy(islbm_solver* s);
+
+/* f = f_eq(rho0, u0 + delta), delta_c ~ U(-amp, amp) from SplitMix64(seed) at counter
+ * ((j*nz + k)*nx + i)*3 + c; macroscopic fields set to (rho0, u0 + delta) and the body
+ * force reset to 8 nu Uref / LZ^2 * 5 (InitialUsingDftFunc, initialization.h:6-49). */
+int islbm_init_equilibrium(islbm_solver* s, double rho0, const double u0[3], double amp, uint64_t seed);
+
+/* Host <-> device transfer of the current populations (FP64 host view, [19][nz][ny][nx])
+ * and macroscopic fields ([nz][ny][nx]; any pointer may be NULL).  Replace
+ * SendDataToGPU / SendDataToCPU (communication.h:245-273). */
+int islbm_upload_f(islbm_solver* s, const double* f);
+int islbm_download_f(islbm_solver* s, double* f);
+int islbm_download_macros(islbm_solver* s, double* rho, double* u, double* v, double* w);
+
+/* Advances nsteps steps; each step is one Launch_CollisionStreaming (evolution.h:842-1001):
+ * bulk-flux accumulation, the fused stream-collide kernel, and every ndtfrc steps the
+ * forcing update (Launch_ModifyForcingTerm, evolution.h:1003-1042) on the device.
+ * Work is queued on the solver's stream; islbm_sync waits for it. */
+int islbm_step(islbm_solver* s, int32_t nsteps);
+int islbm_sync(islbm_solver* s);
+
+/* Launch_ModifyForcingTerm (evolution.h:1003-1042) now, on the device. */
+int islbm_modify_forcing(islbm_solver* s);
+int islbm_get_force(islbm_solver* s, double* force);
+int islbm_set_force(islbm_solver* s, double force);
+/* rho_modify added to F0 before the moments (evolution.h:597). */
+int islbm_set_rho_modify(islbm_solver* s, double drho);
+/* Current value of the bulk-flux accumulator (sum of v*dx*dz over the probe plane). */
+int islbm_get_flux_sum(islbm_solver* s, double* sum);
+
+/* Derived scalars: [dt, tau, nu, uref, a, dx, dy, force0]. */
+int islbm_scalars(islbm_solver* s, double out[8]);
+/* Steps taken since creation / the last init. */
+int islbm_step_count(islbm_solver* s, int64_t* steps);
+/* Kernel in use (ISLBM_KERNEL_PERNODE or ISLBM_KERNEL_TILED). */
+int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel);
+
+/* Geometry table in unique-node form, as FP64 (names and layouts as the oracle's:
+ * x y xi z xw0 xw2 yw0 yw2 xiw3..xiw18 bfl_flag* bfl_q* bfl_x37 ... bfl_xi16).
+ * Returns the count written, or a negative error (-count needed if cap is too small). */
+int islbm_get_table(islbm_solver* s, const char* name, double* out, int32_t cap);
+
+/* Timing of nsteps queued steps with HIP events on the solver's stream:
+ * out[0] = ms for all nsteps, out[1] = summed ms of the stream-collide kernel launches
+ * alone (events recorded around each launch on the stream it runs on). */
+int islbm_time_steps(islbm_solver* s, int32_t nsteps, double out[2]);
+
+const char* islbm_last_error(void);
+
+/* ---------------------------------------------------------------------------------- */
+/* Reference-named entry points (evolution.h:842, :1003).  They drive a process-wide   */
+/* default solver created by islbm_ref_init and take the reference's padded FP64        */
+/* population layout: 19 device arrays of (NX+7) * (NY+7) * (NZ+6) doubles, index      */
+/* J*(NX+7)*(NZ+6) + K*(NX+7) + I, unique node (i,j,k) at (I,J,K) = (i+3, j+3, k+3)     */
+/* (variables.h:16-19 with jp = 1, evolution.h:405).  After each call the ghost and     */
+/* periodic-duplicate layers of f_new and of the padded u, v, w, rho are filled as      */
+/* periodicSW and the single-rank halo exchange leave them (evolution.h:732-777,        */
+/* 980-1000).                                                                            */
+/* ---------------------------------------------------------------------------------- */
+int islbm_ref_init(const islbm_params* p);
+int islbm_ref_shutdown(void);
+/* Device pointers of the padded macroscopic arrays u, v, w, rho_d and of Force_d,
+ * rho_modify_d (memory.h:56-60, 155-159). */
+int islbm_ref_device_arrays(double** u, double** v, double** w, double** rho, double** force, double** rho_modify);
+void Launch_CollisionStreaming(double* f_old[19], double* f_new[19]);
+void Launch_ModifyForcingTerm(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

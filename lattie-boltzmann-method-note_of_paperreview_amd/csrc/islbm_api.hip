@@ -1,0 +1,824 @@
+// islbm_api.hip -- C ABI of the ISLBM periodic-hill solver (declared in include/islbm.h).
+//
+// Replaces the reference's global-state set-up and step orchestration: AllocateMemory
+// (memory.h:51-171), the initialization.h set-up routines, InitialUsingDftFunc
+// (initialization.h:6-49), SendDataToGPU/CPU (communication.h:245-273),
+// Launch_CollisionStreaming (evolution.h:842-1001) and Launch_ModifyForcingTerm
+// (evolution.h:1003-1042).  All state of a solve lives in one islbm_solver and stays
+// resident on its GPU; there is no halo exchange because the domain is not split.
+#include "islbm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace islbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) return fail(ISLBM_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                        \
+    } while (0)
+
+size_t store_bytes(int precision) { return precision == ISLBM_FP64 ? 8 : (precision == ISLBM_FP32 ? 4 : 2); }
+size_t compute_bytes(int precision) { return precision == ISLBM_FP64 ? 8 : 4; }
+
+Store store_of(int precision)
+{
+    switch (precision) {
+    case ISLBM_FP64: return Store::F64;
+    case ISLBM_FP32: return Store::F32;
+    case ISLBM_BF16: return Store::BF16;
+    default: return Store::FP16;
+    }
+}
+
+// 16-bit deviation scale per direction: (f - w_q) / w_q is O(Ma) for every q.
+float dev_scale(int precision, int q) { return precision >= ISLBM_BF16 ? (float)(1.0 / wq(q)) : 1.0f; }
+
+uint16_t f32_to_bf16_rne(float v)
+{
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    const uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+float bf16_to_f32(uint16_t h)
+{
+    uint32_t b = (uint32_t)h << 16;
+    float v;
+    std::memcpy(&v, &b, 4);
+    return v;
+}
+
+size_t table_bytes(const islbm_params& p)
+{
+    const size_t cb = compute_bytes(p.precision);
+    const size_t nx = p.nx, ny = p.ny, nz = p.nz;
+    size_t b = 0;
+    b += 2 * nx * 7 * cb + 2 * ny * 7 * cb;          // x, y weights
+    b += nz * ny * NXI * 7 * cb;                     // xi weights
+    b += NBFL * 2 * ny * (sizeof(int) + cb);         // BFL flags, q
+    b += (size_t)NBW * 2 * ny * 7 * cb;              // BFL weights
+    b += nz * ny * cb;                               // LES filter width
+    b += nz * nx * 8;                                // flux weights
+    b += 8 * 8;                                      // scalars
+    return b;
+}
+
+size_t state_bytes(const islbm_params& p)
+{
+    const size_t n = (size_t)p.nx * p.ny * p.nz;
+    return 2 * NQ * n * store_bytes(p.precision) + 4 * n * compute_bytes(p.precision) + table_bytes(p);
+}
+
+__global__ void k_cast_f64(const double* in, float* out, long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = (float)in[t];
+}
+
+}  // namespace
+
+struct islbm_solver {
+    islbm_params p{};
+    HostGeom g;
+    Store store = Store::F64;
+    bool fp64 = true;
+    int kernel = ISLBM_KERNEL_PERNODE;
+    hipStream_t stream = nullptr;
+    long long n = 0;
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;
+    void* mac[4] = {nullptr, nullptr, nullptr, nullptr};
+    // tables (compute precision)
+    void *xw0 = nullptr, *xw2 = nullptr, *yw0 = nullptr, *yw2 = nullptr, *xiw = nullptr;
+    void *bq = nullptr, *bw = nullptr, *les = nullptr;
+    int* bflag = nullptr;
+    double* fluxw = nullptr;
+    double* scal = nullptr;  // [0] force, [1] rho_modify, [2] flux accumulator
+    DevTables<double> td{};
+    DevTables<float> tf{};
+    float scale[NQ];
+    long long steps = 0;
+    bool auto_force = true;  // run the forcing controller every ndtfrc steps
+    size_t bytes = 0;
+    std::vector<hipEvent_t> evpool;
+};
+
+namespace {
+
+void free_all(islbm_solver* s)
+{
+    void* ptrs[] = {s->f[0], s->f[1], s->mac[0], s->mac[1], s->mac[2], s->mac[3], s->xw0, s->xw2, s->yw0,
+                    s->yw2, s->xiw, s->bq, s->bw, s->les, s->bflag, s->fluxw, s->scal};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+template <typename C>
+int upload_table(islbm_solver* s, void** dst, const std::vector<double>& src)
+{
+    std::vector<C> tmp(src.begin(), src.end());
+    HIPCHK(hipMalloc(dst, tmp.size() * sizeof(C) + 8));
+    HIPCHK(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(C), hipMemcpyHostToDevice));
+    return ISLBM_OK;
+}
+
+template <typename C>
+int upload_tables(islbm_solver* s)
+{
+    const HostGeom& g = s->g;
+    int rc;
+    if ((rc = upload_table<C>(s, &s->xw0, g.xw0)) || (rc = upload_table<C>(s, &s->xw2, g.xw2)) ||
+        (rc = upload_table<C>(s, &s->yw0, g.yw0)) || (rc = upload_table<C>(s, &s->yw2, g.yw2)) ||
+        (rc = upload_table<C>(s, &s->bq, g.bq)) || (rc = upload_table<C>(s, &s->bw, g.bw)) ||
+        (rc = upload_table<C>(s, &s->les, g.les_delta)))
+        return rc;
+    // BFL branch codes decided in FP64 (see apply_walls)
+    std::vector<int> code(g.bflag.size());
+    for (size_t t = 0; t < code.size(); ++t)
+        code[t] = !g.bflag[t] ? 0 : (g.bq[t] > 0.5 ? 1 : (g.bq[t] < 0.5 ? 2 : 3));
+    HIPCHK(hipMalloc((void**)&s->bflag, code.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(s->bflag, code.data(), code.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void**)&s->fluxw, g.fluxw.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(s->fluxw, g.fluxw.data(), g.fluxw.size() * sizeof(double), hipMemcpyHostToDevice));
+
+    // per-node xi weights: FP64 on the device, then stored in compute precision
+    const size_t nxi = (size_t)g.nz * g.ny * NXI * 7;
+    double *d64 = nullptr, *dzc = nullptr, *dy = nullptr, *dxi = nullptr;
+    HIPCHK(hipMalloc((void**)&d64, nxi * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&dzc, g.zc.size() * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&dy, g.y.size() * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&dxi, g.xi.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(dzc, g.zc.data(), g.zc.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dy, g.y.data(), g.y.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dxi, g.xi.data(), g.xi.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(launch_xi_weights(s->stream, d64, dzc, dy, dxi, g.nx, g.ny, g.nz, s->p.ly, s->p.lz, s->p.lxi,
+                             s->p.hill_alpha, s->p.flat, g.minsize));
+    if (sizeof(C) == 8) {
+        s->xiw = d64;
+        d64 = nullptr;
+    } else {
+        HIPCHK(hipMalloc(&s->xiw, nxi * sizeof(float)));
+        hipLaunchKernelGGL(k_cast_f64, dim3((unsigned)((nxi + 255) / 256)), dim3(256), 0, s->stream, d64,
+                           (float*)s->xiw, (long long)nxi);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (double* q : {d64, dzc, dy, dxi})
+        if (q) HIPCHK(hipFree(q));
+
+    DevTables<C> t;
+    t.xw0 = (const C*)s->xw0; t.xw2 = (const C*)s->xw2; t.yw0 = (const C*)s->yw0; t.yw2 = (const C*)s->yw2;
+    t.xiw = (const C*)s->xiw; t.bflag = s->bflag; t.bq = (const C*)s->bq; t.bw = (const C*)s->bw;
+    t.les_delta = (const C*)s->les;
+    if constexpr (sizeof(C) == 8)
+        s->td = t;
+    else
+        s->tf = t;
+    return ISLBM_OK;
+}
+
+// host FP64 view <-> device storage of one population buffer
+int put_f(islbm_solver* s, int buf, const double* f)
+{
+    const long long n = s->n;
+    const size_t tot = (size_t)NQ * n;
+    if (s->store == Store::F64) {
+        HIPCHK(hipMemcpy(s->f[buf], f, tot * 8, hipMemcpyHostToDevice));
+        return ISLBM_OK;
+    }
+    if (s->store == Store::F32) {
+        std::vector<float> h(tot);
+        for (int q = 0; q < NQ; ++q)
+            for (long long t = 0; t < n; ++t) h[q * n + t] = (float)(f[q * n + t] - wq(q));
+        HIPCHK(hipMemcpy(s->f[buf], h.data(), tot * 4, hipMemcpyHostToDevice));
+        return ISLBM_OK;
+    }
+    std::vector<uint16_t> h(tot);
+    for (int q = 0; q < NQ; ++q) {
+        const double sc = s->scale[q];
+        for (long long t = 0; t < n; ++t) {
+            const float v = (float)((f[q * n + t] - wq(q)) * sc);
+            if (s->store == Store::BF16) {
+                h[q * n + t] = f32_to_bf16_rne(v);
+            } else {
+                _Float16 hv = (_Float16)v;
+                std::memcpy(&h[q * n + t], &hv, 2);
+            }
+        }
+    }
+    HIPCHK(hipMemcpy(s->f[buf], h.data(), tot * 2, hipMemcpyHostToDevice));
+    return ISLBM_OK;
+}
+
+int get_f(islbm_solver* s, int buf, double* f)
+{
+    const long long n = s->n;
+    const size_t tot = (size_t)NQ * n;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->store == Store::F64) {
+        HIPCHK(hipMemcpy(f, s->f[buf], tot * 8, hipMemcpyDeviceToHost));
+        return ISLBM_OK;
+    }
+    if (s->store == Store::F32) {
+        std::vector<float> h(tot);
+        HIPCHK(hipMemcpy(h.data(), s->f[buf], tot * 4, hipMemcpyDeviceToHost));
+        for (int q = 0; q < NQ; ++q)
+            for (long long t = 0; t < n; ++t) f[q * n + t] = (double)h[q * n + t] + wq(q);
+        return ISLBM_OK;
+    }
+    std::vector<uint16_t> h(tot);
+    HIPCHK(hipMemcpy(h.data(), s->f[buf], tot * 2, hipMemcpyDeviceToHost));
+    for (int q = 0; q < NQ; ++q) {
+        const double isc = 1.0 / (double)s->scale[q];
+        for (long long t = 0; t < n; ++t) {
+            float v;
+            if (s->store == Store::BF16) {
+                v = bf16_to_f32(h[q * n + t]);
+            } else {
+                _Float16 hv;
+                std::memcpy(&hv, &h[q * n + t], 2);
+                v = (float)hv;
+            }
+            f[q * n + t] = (double)v * isc + wq(q);
+        }
+    }
+    return ISLBM_OK;
+}
+
+int put_macros(islbm_solver* s, const double* const* m4)
+{
+    const long long n = s->n;
+    for (int a = 0; a < 4; ++a) {
+        if (s->fp64) {
+            HIPCHK(hipMemcpy(s->mac[a], m4[a], n * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> h(m4[a], m4[a] + n);
+            HIPCHK(hipMemcpy(s->mac[a], h.data(), n * 4, hipMemcpyHostToDevice));
+        }
+    }
+    return ISLBM_OK;
+}
+
+StepLaunch make_launch(islbm_solver* s)
+{
+    StepLaunch L{};
+    L.store = s->store;
+    L.fp64 = s->fp64;
+    L.kernel = s->kernel;
+    L.g = LaunchGeom{s->p.nx, s->p.ny, s->p.nz, s->n};
+    L.fo = s->f[s->cur];
+    L.fn = s->f[1 - s->cur];
+    L.rho = s->mac[0];
+    L.u = s->mac[1];
+    L.v = s->mac[2];
+    L.w = s->mac[3];
+    L.tables = s->fp64 ? (const void*)&s->td : (const void*)&s->tf;
+    L.force = s->scal + 0;
+    L.rho_modify = s->scal + 1;
+    L.tau = s->g.tau;
+    L.dt = s->g.dt;
+    L.cs_smag = s->p.cs_smag;
+    L.scale = (s->store == Store::BF16 || s->store == Store::FP16) ? s->scale : nullptr;
+    return L;
+}
+
+int enqueue_force_update(islbm_solver* s)
+{
+    const double beta = (0.001 > 3.0 / s->p.re) ? 0.001 : 3.0 / s->p.re;
+    HIPCHK(launch_force_update(s->stream, s->scal + 0, s->scal + 2, 1.0 / (double)(s->p.lx * (s->p.lz - 1.0)),
+                               s->p.ndtfrc, beta, s->g.uref));
+    return ISLBM_OK;
+}
+
+// one Launch_CollisionStreaming; ev (optional) brackets the stream-collide launch
+int enqueue_step(islbm_solver* s, hipEvent_t ev0, hipEvent_t ev1)
+{
+    HIPCHK(launch_flux_probe(s->stream, s->mac[2], s->fp64, s->fluxw, s->scal + 2, s->p.nx, s->p.ny, s->p.nz,
+                             s->g.flux_plane));
+    if (ev0) HIPCHK(hipEventRecord(ev0, s->stream));
+    StepLaunch L = make_launch(s);
+    HIPCHK(launch_stream_collide(s->stream, L));
+    if (ev1) HIPCHK(hipEventRecord(ev1, s->stream));
+    s->cur = 1 - s->cur;
+    s->steps++;
+    if (s->auto_force && s->steps % s->p.ndtfrc == 0) return enqueue_force_update(s);
+    return ISLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int islbm_default_params(islbm_params* p)
+{
+    if (!p) return fail(ISLBM_ERR_ARG, "null params");
+    std::memset(p, 0, sizeof *p);
+    p->nx = 32; p->ny = 128; p->nz = 64;              // variables.h:10-12 (global NY)
+    p->lx = 4.5; p->ly = 9.0; p->lz = 3.036;          // variables.h:6-8
+    p->cfl = 0.6;                                      // variables.h:22
+    p->lxi = 10.0;                                     // variables.h:30
+    p->hill_alpha = 1.0;
+    p->flat = 0;
+    p->mesh_mode = ISLBM_MESH_BISECT;
+    p->stretch = 2.0;
+    p->tau = 0.6833;                                   // variables.h:95
+    p->re = 50.0;                                      // variables.h:41
+    p->u_ref = 0.0;
+    p->cs_smag = 0.0;
+    p->ndtfrc = 10000;                                 // variables.h:48
+    p->precision = ISLBM_FP64;
+    p->kernel = ISLBM_KERNEL_AUTO;
+    p->device = 0;
+    return ISLBM_OK;
+}
+
+int islbm_required_bytes(const islbm_params* p, size_t* bytes)
+{
+    if (!p || !bytes) return fail(ISLBM_ERR_ARG, "null argument");
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0) return fail(ISLBM_ERR_ARG, "non-positive grid");
+    *bytes = state_bytes(*p);
+    return ISLBM_OK;
+}
+
+int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
+{
+    if (!pin || !out) return fail(ISLBM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (pin->precision < ISLBM_FP64 || pin->precision > ISLBM_FP16) return fail(ISLBM_ERR_ARG, "unknown precision");
+    if (pin->kernel < ISLBM_KERNEL_AUTO || pin->kernel > ISLBM_KERNEL_TILED) return fail(ISLBM_ERR_ARG, "unknown kernel");
+    islbm_solver* s = new islbm_solver();
+    s->p = *pin;
+    std::string msg = build_host_geometry(s->p, s->g);
+    if (!msg.empty()) {
+        delete s;
+        return fail(ISLBM_ERR_ARG, "%s", msg.c_str());
+    }
+    s->bytes = state_bytes(s->p);
+    if (budget && s->bytes > budget) {
+        delete s;
+        return fail(ISLBM_ERR_BUDGET, "solve needs %zu bytes, budget %zu", s->bytes, budget);
+    }
+    s->n = (long long)s->p.nx * s->p.ny * s->p.nz;
+    s->store = store_of(s->p.precision);
+    s->fp64 = s->p.precision == ISLBM_FP64;
+    for (int q = 0; q < NQ; ++q) s->scale[q] = dev_scale(s->p.precision, q);
+    const LaunchGeom lg{s->p.nx, s->p.ny, s->p.nz, s->n};
+    if (s->p.kernel == ISLBM_KERNEL_AUTO)
+        s->kernel = tiled_supported(lg, s->store, s->fp64) ? ISLBM_KERNEL_TILED : ISLBM_KERNEL_PERNODE;
+    else
+        s->kernel = s->p.kernel;
+    if (s->kernel == ISLBM_KERNEL_TILED && !tiled_supported(lg, s->store, s->fp64)) {
+        delete s;
+        return fail(ISLBM_ERR_UNSUPPORTED, "tiled kernel does not support this grid/precision");
+    }
+    auto bail = [&](int rc) {
+        free_all(s);
+        delete s;
+        return rc;
+    };
+    if (hipSetDevice(s->p.device) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipSetDevice(%d) failed", s->p.device));
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(ISLBM_ERR_HIP, "hipStreamCreate failed"));
+    const size_t fb = (size_t)NQ * s->n * store_bytes(s->p.precision);
+    const size_t mb = (size_t)s->n * compute_bytes(s->p.precision);
+    for (int b = 0; b < 2; ++b) {
+        if (hipMalloc(&s->f[b], fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc f (%zu bytes)", fb));
+        if (hipMemset(s->f[b], 0, fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset f"));
+    }
+    for (int a = 0; a < 4; ++a) {
+        if (hipMalloc(&s->mac[a], mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc macros"));
+        if (hipMemset(s->mac[a], 0, mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset macros"));
+    }
+    if (hipMalloc((void**)&s->scal, 8 * sizeof(double)) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc scalars"));
+    const double sc[8] = {s->g.force0, 0.0, 0.0, 0, 0, 0, 0, 0};
+    if (hipMemcpy(s->scal, sc, sizeof sc, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "scalars"));
+    int rc = s->fp64 ? upload_tables<double>(s) : upload_tables<float>(s);
+    if (rc) return bail(rc);
+    *out = s;
+    return ISLBM_OK;
+}
+
+int islbm_destroy(islbm_solver* s)
+{
+    if (!s) return ISLBM_OK;
+    (void)hipSetDevice(s->p.device);
+    (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return ISLBM_OK;
+}
+
+int islbm_init_equilibrium(islbm_solver* s, double rho0, const double u0[3], double amp, uint64_t seed)
+{
+    if (!s || !u0) return fail(ISLBM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const long long n = s->n;
+    std::vector<double> f((size_t)NQ * n), m((size_t)4 * n);
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i) {
+                const long long id = ((long long)k * ny + j) * nx + i;
+                const uint64_t base = (((uint64_t)j * nz + k) * nx + i) * 3;
+                double uu[3];
+                for (int c = 0; c < 3; ++c) {
+                    double d = 0.0;
+                    if (amp != 0.0) {
+                        // SplitMix64 finaliser on the counter
+                        uint64_t z = seed + (base + c + 1) * 0x9E3779B97F4A7C15ULL;
+                        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+                        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+                        z = z ^ (z >> 31);
+                        d = amp * (2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
+                    }
+                    uu[c] = u0[c] + d;
+                }
+                const double udot = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
+                // initialization.h:36-41
+                f[id] = wq(0) * rho0 * (1.0 - 1.5 * udot);
+                for (int q = 1; q < NQ; ++q) {
+                    const double eu = cx(q) * uu[0] + cy(q) * uu[1] + cz(q) * uu[2];
+                    f[(size_t)q * n + id] = wq(q) * rho0 * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * udot);
+                }
+                m[id] = rho0; m[n + id] = uu[0]; m[2 * n + id] = uu[1]; m[3 * n + id] = uu[2];
+            }
+    s->cur = 0;
+    int rc = put_f(s, 0, f.data());
+    if (rc) return rc;
+    const double* m4[4] = {m.data(), m.data() + n, m.data() + 2 * n, m.data() + 3 * n};
+    if ((rc = put_macros(s, m4))) return rc;
+    const double sc[3] = {s->g.force0, 0.0, 0.0};
+    HIPCHK(hipMemcpy(s->scal, sc, sizeof sc, hipMemcpyHostToDevice));
+    s->steps = 0;
+    return ISLBM_OK;
+}
+
+int islbm_upload_f(islbm_solver* s, const double* f)
+{
+    if (!s || !f) return fail(ISLBM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return put_f(s, s->cur, f);
+}
+
+int islbm_download_f(islbm_solver* s, double* f)
+{
+    if (!s || !f) return fail(ISLBM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    return get_f(s, s->cur, f);
+}
+
+int islbm_download_macros(islbm_solver* s, double* rho, double* u, double* v, double* w)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double* outs[4] = {rho, u, v, w};
+    std::vector<float> h;
+    for (int a = 0; a < 4; ++a) {
+        if (!outs[a]) continue;
+        if (s->fp64) {
+            HIPCHK(hipMemcpy(outs[a], s->mac[a], s->n * 8, hipMemcpyDeviceToHost));
+        } else {
+            h.resize(s->n);
+            HIPCHK(hipMemcpy(h.data(), s->mac[a], s->n * 4, hipMemcpyDeviceToHost));
+            for (long long t = 0; t < s->n; ++t) outs[a][t] = h[t];
+        }
+    }
+    return ISLBM_OK;
+}
+
+int islbm_step(islbm_solver* s, int32_t nsteps)
+{
+    if (!s || nsteps < 0) return fail(ISLBM_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    for (int t = 0; t < nsteps; ++t) {
+        int rc = enqueue_step(s, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    return ISLBM_OK;
+}
+
+int islbm_sync(islbm_solver* s)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ISLBM_OK;
+}
+
+int islbm_modify_forcing(islbm_solver* s)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    return enqueue_force_update(s);
+}
+
+int islbm_get_force(islbm_solver* s, double* force)
+{
+    if (!s || !force) return fail(ISLBM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(force, s->scal + 0, sizeof(double), hipMemcpyDeviceToHost));
+    return ISLBM_OK;
+}
+
+int islbm_set_force(islbm_solver* s, double force)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(s->scal + 0, &force, sizeof(double), hipMemcpyHostToDevice));
+    return ISLBM_OK;
+}
+
+int islbm_set_rho_modify(islbm_solver* s, double drho)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(s->scal + 1, &drho, sizeof(double), hipMemcpyHostToDevice));
+    return ISLBM_OK;
+}
+
+int islbm_get_flux_sum(islbm_solver* s, double* sum)
+{
+    if (!s || !sum) return fail(ISLBM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(sum, s->scal + 2, sizeof(double), hipMemcpyDeviceToHost));
+    return ISLBM_OK;
+}
+
+int islbm_scalars(islbm_solver* s, double out[8])
+{
+    if (!s || !out) return fail(ISLBM_ERR_ARG, "null argument");
+    const HostGeom& g = s->g;
+    const double v[8] = {g.dt, g.tau, g.nu, g.uref, g.a, g.dx, g.dy, g.force0};
+    std::memcpy(out, v, sizeof v);
+    return ISLBM_OK;
+}
+
+int islbm_step_count(islbm_solver* s, int64_t* steps)
+{
+    if (!s || !steps) return fail(ISLBM_ERR_ARG, "null argument");
+    *steps = s->steps;
+    return ISLBM_OK;
+}
+
+int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel)
+{
+    if (!s || !kernel) return fail(ISLBM_ERR_ARG, "null argument");
+    *kernel = s->kernel;
+    return ISLBM_OK;
+}
+
+int islbm_get_table(islbm_solver* s, const char* name, double* out, int32_t cap)
+{
+    if (!s || !name || !out) return fail(ISLBM_ERR_ARG, "null argument");
+    const HostGeom& g = s->g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    std::vector<double> v;
+    const std::string nm(name);
+    if (nm == "x") v = g.x;
+    else if (nm == "y") v = g.y;
+    else if (nm == "xi") v = g.xi;
+    else if (nm == "z") v = g.zc;
+    else if (nm == "xw0") v = g.xw0;
+    else if (nm == "xw2") v = g.xw2;
+    else if (nm == "yw0") v = g.yw0;
+    else if (nm == "yw2") v = g.yw2;
+    else if (nm.rfind("xiw", 0) == 0) {
+        static const char* fam[NXI] = {"xiw3", "xiw4", "xiw5", "xiw6", "xiw15", "xiw16", "xiw17", "xiw18"};
+        int f = -1;
+        for (int t = 0; t < NXI; ++t)
+            if (nm == fam[t]) f = t;
+        if (f < 0) return fail(ISLBM_ERR_ARG, "unknown table %s", name);
+        const size_t nxi = (size_t)nz * ny * NXI * 7;
+        std::vector<double> all(nxi);
+        HIPCHK(hipSetDevice(s->p.device));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (s->fp64) {
+            HIPCHK(hipMemcpy(all.data(), s->xiw, nxi * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> h(nxi);
+            HIPCHK(hipMemcpy(h.data(), s->xiw, nxi * 4, hipMemcpyDeviceToHost));
+            for (size_t t = 0; t < nxi; ++t) all[t] = h[t];
+        }
+        v.resize((size_t)nz * ny * 7);
+        for (size_t kj = 0; kj < (size_t)nz * ny; ++kj)
+            for (int m = 0; m < 7; ++m) v[kj * 7 + m] = all[(kj * NXI + f) * 7 + m];
+    } else if (nm.rfind("bfl_flag", 0) == 0 || nm.rfind("bfl_q", 0) == 0) {
+        const bool isq = nm.rfind("bfl_q", 0) == 0;
+        const std::string suf = nm.substr(isq ? 5 : 8);
+        const int b = suf == "3" ? B3 : suf == "4" ? B4 : suf == "15" ? B15 : suf == "16" ? B16 : -1;
+        if (b < 0) return fail(ISLBM_ERR_ARG, "unknown table %s", name);
+        const size_t nb = 2 * (size_t)ny;
+        v.resize(nb);
+        for (size_t t = 0; t < nb; ++t) v[t] = isq ? g.bq[b * nb + t] : (double)g.bflag[b * nb + t];
+    } else if (nm.rfind("bfl_", 0) == 0) {
+        static const char* wn[NBW] = {"bfl_x37", "bfl_x38", "bfl_y378", "bfl_xi378", "bfl_x49", "bfl_x410",
+                                      "bfl_y4910", "bfl_xi4910", "bfl_y15", "bfl_xi15", "bfl_y16", "bfl_xi16"};
+        int b = -1;
+        for (int t = 0; t < NBW; ++t)
+            if (nm == wn[t]) b = t;
+        if (b < 0) return fail(ISLBM_ERR_ARG, "unknown table %s", name);
+        const size_t cnt = 2 * (size_t)ny * 7;
+        v.assign(g.bw.begin() + b * cnt, g.bw.begin() + (b + 1) * cnt);
+    } else if (nm == "fluxw") {
+        v = g.fluxw;
+    } else {
+        return fail(ISLBM_ERR_ARG, "unknown table %s", name);
+    }
+    (void)nx;
+    if ((int64_t)v.size() > cap) return -(int)v.size();
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return (int)v.size();
+}
+
+int islbm_time_steps(islbm_solver* s, int32_t nsteps, double out[2])
+{
+    if (!s || !out || nsteps <= 0) return fail(ISLBM_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    const size_t need = 2 * (size_t)nsteps + 2;
+    while (s->evpool.size() < need) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        s->evpool.push_back(e);
+    }
+    hipEvent_t* ev = s->evpool.data();
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipEventRecord(ev[0], s->stream));
+    for (int t = 0; t < nsteps; ++t) {
+        int rc = enqueue_step(s, ev[2 + 2 * t], ev[3 + 2 * t]);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(ev[1], s->stream));
+    HIPCHK(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    out[0] = ms;
+    double ksum = 0.0;
+    for (int t = 0; t < nsteps; ++t) {
+        float k = 0.f;
+        HIPCHK(hipEventElapsedTime(&k, ev[2 + 2 * t], ev[3 + 2 * t]));
+        ksum += k;
+    }
+    out[1] = ksum;
+    return ISLBM_OK;
+}
+
+const char* islbm_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Reference-named entry points on the reference's padded FP64 layout
+// ------------------------------------------------------------------------------------
+namespace {
+islbm_solver* g_ref = nullptr;
+double* g_ref_mac[4] = {nullptr, nullptr, nullptr, nullptr};  // padded u, v, w, rho_d
+
+void ref_release()
+{
+    for (double*& p : g_ref_mac)
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+    if (g_ref) {
+        islbm_destroy(g_ref);
+        g_ref = nullptr;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int islbm_ref_init(const islbm_params* p)
+{
+    if (!p) return fail(ISLBM_ERR_ARG, "null params");
+    if (p->precision != ISLBM_FP64)
+        return fail(ISLBM_ERR_UNSUPPORTED, "the reference-named entry points take FP64 populations");
+    ref_release();
+    islbm_params q = *p;
+    q.kernel = ISLBM_KERNEL_PERNODE;
+    int rc = islbm_create(&q, 0, &g_ref);
+    if (rc) return rc;
+    g_ref->auto_force = false;  // the reference's main calls Launch_ModifyForcingTerm itself
+    const size_t padded = (size_t)(p->nx + 7) * (p->ny + 7) * (p->nz + 6);
+    for (double*& m : g_ref_mac) {
+        HIPCHK(hipMalloc((void**)&m, padded * sizeof(double)));
+        HIPCHK(hipMemset(m, 0, padded * sizeof(double)));
+    }
+    return ISLBM_OK;
+}
+
+int islbm_ref_shutdown(void)
+{
+    ref_release();
+    return ISLBM_OK;
+}
+
+int islbm_ref_device_arrays(double** u, double** v, double** w, double** rho, double** force, double** rho_modify)
+{
+    if (!g_ref) return fail(ISLBM_ERR_STATE, "islbm_ref_init has not been called");
+    if (u) *u = g_ref_mac[0];
+    if (v) *v = g_ref_mac[1];
+    if (w) *w = g_ref_mac[2];
+    if (rho) *rho = g_ref_mac[3];
+    if (force) *force = g_ref->scal + 0;
+    if (rho_modify) *rho_modify = g_ref->scal + 1;
+    return ISLBM_OK;
+}
+
+// Launch_CollisionStreaming (evolution.h:842-1001): f_old -> f_new for every unique node,
+// then the ghost and duplicate layers of f_new and of u, v, w, rho_d are refreshed.
+// Errors are recorded for islbm_last_error (the reference exits the process instead).
+void Launch_CollisionStreaming(double* f_old[19], double* f_new[19])
+{
+    islbm_solver* s = g_ref;
+    if (!s || !f_old || !f_new) {
+        fail(ISLBM_ERR_STATE, "Launch_CollisionStreaming: islbm_ref_init has not been called");
+        return;
+    }
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const long long n = s->n;
+    if (hipSetDevice(s->p.device) != hipSuccess) {
+        fail(ISLBM_ERR_HIP, "hipSetDevice failed");
+        return;
+    }
+    PtrSet pad{}, uni{};
+    for (int q = 0; q < NQ; ++q) {
+        pad.p[q] = f_old[q];
+        uni.p[q] = (double*)s->f[s->cur] + (long long)q * n;
+    }
+    // macroscopic state (u, v, w, rho_d): the flux probe reads v of the previous step
+    const int mac_of[4] = {1, 2, 3, 0};  // unique order rho,u,v,w <- padded order u,v,w,rho
+    for (int a = 0; a < 4; ++a) {
+        pad.p[NQ + a] = g_ref_mac[a];
+        uni.p[NQ + a] = (double*)s->mac[mac_of[a]];
+    }
+    hipError_t e = launch_padded_to_unique(s->stream, pad, uni, nx, ny, nz, NQ);
+    if (e == hipSuccess) {
+        PtrSet pm{}, um{};
+        for (int a = 0; a < 4; ++a) { pm.p[a] = pad.p[NQ + a]; um.p[a] = uni.p[NQ + a]; }
+        e = launch_padded_to_unique(s->stream, pm, um, nx, ny, nz, 4);
+    }
+    if (e != hipSuccess) {
+        fail(ISLBM_ERR_HIP, "Launch_CollisionStreaming gather: %s", hipGetErrorString(e));
+        return;
+    }
+    if (enqueue_step(s, nullptr, nullptr) != ISLBM_OK) return;
+    PtrSet pn{}, un{};
+    for (int q = 0; q < NQ; ++q) {
+        pn.p[q] = f_new[q];
+        un.p[q] = (double*)s->f[s->cur] + (long long)q * n;
+    }
+    for (int a = 0; a < 4; ++a) {
+        pn.p[NQ + a] = g_ref_mac[a];
+        un.p[NQ + a] = (double*)s->mac[mac_of[a]];
+    }
+    e = launch_unique_to_padded(s->stream, un, pn, nx, ny, nz, NQ + 4);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) fail(ISLBM_ERR_HIP, "Launch_CollisionStreaming scatter: %s", hipGetErrorString(e));
+}
+
+// Launch_ModifyForcingTerm (evolution.h:1003-1042) on the device; single rank.
+void Launch_ModifyForcingTerm(void)
+{
+    if (!g_ref) {
+        fail(ISLBM_ERR_STATE, "Launch_ModifyForcingTerm: islbm_ref_init has not been called");
+        return;
+    }
+    if (hipSetDevice(g_ref->p.device) != hipSuccess || enqueue_force_update(g_ref) != ISLBM_OK) return;
+    if (hipStreamSynchronize(g_ref->stream) != hipSuccess) fail(ISLBM_ERR_HIP, "Launch_ModifyForcingTerm sync");
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// islbm_internal.h -- host-side geometry, device state and kernel launchers of the solver.
+#pragma once
+
+#include "../../include/islbm.h"
+#include "islbm_common.h"
+
+#include <string>
+#include <vector>
+
+namespace islbm {
+
+// FP64 geometry of one solve in unique-node form (what AllocateMemory and the
+// initialization.h routines produce for one rank, minus ghosts and duplicates).
+struct HostGeom {
+    int nx = 0, ny = 0, nz = 0;
+    double minsize = 0, dt = 0, tau = 0, nu = 0, uref = 0, a = 0, dx = 0, dy = 0, force0 = 0;
+    std::vector<double> x, y, xi;        // x[nx] y[ny] xi[nz]
+    std::vector<double> zc;              // node z, [ny][nz]
+    std::vector<double> xw0, xw2;        // [nx][7]
+    std::vector<double> yw0, yw2;        // [ny][7]
+    std::vector<int> bflag;              // [NBFL][2][ny]
+    std::vector<double> bq;              // [NBFL][2][ny]
+    std::vector<double> bw;              // [NBW][2][ny][7]
+    std::vector<double> fluxw;           // [nz][nx]  dx_i * dz_k on the flux probe plane
+    std::vector<double> les_delta;       // [nz][ny]  (dx dy dz_local)^(1/3)
+    int flux_plane = 0;                  // unique j of the probe plane (y = LY - 3 dy)
+};
+
+// Validates p and fills g (everything except the per-node xi weights, which are computed
+// on the device by launch_xi_weights).  Returns an error message or "".
+std::string build_host_geometry(const islbm_params& p, HostGeom& g);
+
+// Device-resident tables in compute precision C.
+template <typename C>
+struct DevTables {
+    const C* xw0;  // [nx][7]
+    const C* xw2;
+    const C* yw0;  // [ny][7]
+    const C* yw2;
+    const C* xiw;  // [nz][ny][NXI][7]
+    const int* bflag;  // [NBFL][2][ny]
+    const C* bq;       // [NBFL][2][ny]
+    const C* bw;       // [NBW][2][ny][7]
+    const C* les_delta;  // [nz][ny]
+};
+
+// Population storage codec: FP64/FP32 store f directly; 16-bit formats store
+// (f_q - w_q) * scale_q.
+enum class Store { F64, F32, BF16, FP16 };
+
+struct LaunchGeom {
+    int nx, ny, nz;
+    long long n;  // nodes
+};
+
+// Kernel launchers (islbm_kernels.hip).
+hipError_t launch_xi_weights(hipStream_t st, double* out, const double* zc, const double* y,
+                             const double* xi, int nx, int ny, int nz, double ly, double lz,
+                             double lxi, double alpha, int flat, double minsize);
+hipError_t launch_convert_f64(hipStream_t st, const double* in, void* out, long long n, int to_fp32);
+
+struct StepLaunch {
+    Store store;
+    bool fp64;            // compute precision
+    int kernel;           // ISLBM_KERNEL_PERNODE / _TILED
+    LaunchGeom g;
+    const void* fo;
+    void* fn;
+    void* rho;
+    void* u;
+    void* v;
+    void* w;
+    const void* tables;   // DevTables<C>*, host copy passed by value
+    const double* force;
+    const double* rho_modify;
+    double tau, dt, cs_smag;
+    const float* scale;   // per-direction 16-bit scale (device), or null
+};
+hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L);
+bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
+
+hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw,
+                             double* acc, int nx, int ny, int nz, int jplane);
+hipError_t launch_force_update(hipStream_t st, double* force, double* acc, double inv_area,
+                               int ndtfrc, double beta, double uref);
+
+// Reference padded layout (NX6 x NY6 x NZ6, variables.h:16-19 with jp = 1) <-> unique
+// nodes, FP64, for the reference-named entry points.  narr arrays are moved per call.
+struct PtrSet {
+    double* p[NQ + 4];
+};
+hipError_t launch_padded_to_unique(hipStream_t st, const PtrSet& pad, const PtrSet& uni, int nx, int ny, int nz,
+                                   int narr);
+hipError_t launch_unique_to_padded(hipStream_t st, const PtrSet& uni, const PtrSet& pad, int nx, int ny, int nz,
+                                   int narr);
+
+}  // namespace islbm

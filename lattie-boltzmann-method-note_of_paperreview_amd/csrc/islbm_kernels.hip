@@ -1,0 +1,261 @@
+// islbm_kernels.hip -- gfx950 kernels of the ISLBM periodic-hill step.
+//
+//   k_xi_weights        FP64 per-node xi interpolation weights (GetIntrplParameter_Xi,
+//                       initialization.h:185-258), 8 departure families per node.
+//   k_collide_pernode   one thread per node: interpolated streaming of all 19 populations
+//                       (interpolationHillISLBM.h:4-161, evolution.h:451-475), flat-wall
+//                       bounce-back (:482-496), BFL (:505-572), moments, equilibrium and MRT
+//                       with body force (:597-679, MRT_Process.h:4-179), optional
+//                       Smagorinsky relaxation.  x and y periodicity by index wrap, which
+//                       replaces periodicSW (:732-777) and the y halo exchange (:980-994).
+//   k_flux_probe        FP64 bulk-flux sum over the probe plane (AccumulateUbulk, :826-840).
+//   k_force_update      forcing controller (Launch_ModifyForcingTerm, :1003-1042).
+//
+// Population storage is [q][k][j][i].  FP64 stores f itself; FP32 stores f - w_q; the
+// 16-bit formats store (f - w_q) * scale_q.  Reduced-precision kernels compute in the
+// deviation form throughout (rho = 1 + sum), which keeps FP32 cancellation small.
+#include "islbm_internal.h"
+#include "islbm_step.h"
+
+namespace islbm {
+
+// ------------------------------------------------------------------------------------
+// Set-up kernels
+// ------------------------------------------------------------------------------------
+
+__global__ void k_xi_weights(double* __restrict__ out, const double* __restrict__ zc,
+                             const double* __restrict__ y, const double* __restrict__ xi, int ny, int nz,
+                             double ly, double lz, double lxi, double alpha, int flat, double ms)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nz * ny * NXI) return;
+    const int f = t % NXI;
+    const int jk = t / NXI;
+    const int j = jk % ny;
+    const int k = jk / ny;
+    const double pz = zc[(size_t)j * nz + k] + xi_dz(f) * ms;
+    const double py = y[j] + xi_dy(f) * ms;
+    const double H = hill_profile(py, ly, alpha, flat);
+    const double L = lz - H - ms;
+    const double pxi = lxi * (pz - (H + ms / 2.0)) / L;
+    const double* nodes = xi + xi_stencil_start(k, nz);
+    double* o = out + ((size_t)(k * ny + j) * NXI + f) * 7;
+    for (int m = 0; m < 7; ++m) o[m] = lagrange7_weight(pxi, nodes, m);
+}
+
+hipError_t launch_xi_weights(hipStream_t st, double* out, const double* zc, const double* y, const double* xi,
+                             int nx, int ny, int nz, double ly, double lz, double lxi, double alpha, int flat,
+                             double minsize)
+{
+    (void)nx;
+    const int n = nz * ny * NXI;
+    hipLaunchKernelGGL(k_xi_weights, dim3((n + 255) / 256), dim3(256), 0, st, out, zc, y, xi, ny, nz, ly, lz, lxi,
+                       alpha, flat, minsize);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Per-node stream-collide
+// ------------------------------------------------------------------------------------
+
+template <Store ST>
+__global__ __launch_bounds__(256) void k_collide_pernode(StepArgs<ST> a)
+{
+    using CD = Codec<ST>;
+    using C = typename CD::C;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= a.n) return;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int i = (int)(gid % nx);
+    const long long t = gid / nx;
+    const int j = (int)(t % ny);
+    const int k = (int)(t / ny);
+    const long long plane = (long long)nx * ny;
+    const long long N = a.n;
+
+    int xo[7], yo[7];
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+        xo[l] = wrap_index(i - 3 + l, nx);
+        yo[l] = wrap_index(j - 3 + l, ny) * nx;
+    }
+    const long long zb = (long long)xi_stencil_start(k, nz) * plane;
+    const long long row = (long long)k * plane + (long long)j * nx;  // this node's x-row
+    const long long node = row + i;
+    const C* X0 = a.t.xw0 + i * 7;
+    const C* X2 = a.t.xw2 + i * 7;
+    const C* Y0 = a.t.yw0 + j * 7;
+    const C* Y2 = a.t.yw2 + j * 7;
+    const C* XI = a.t.xiw + ((long long)k * ny + j) * (NXI * 7);
+
+    auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * N, a.inv_scale[q]}; };
+
+    C F[NQ];
+    F[0] = pop(0)(node);
+    F[1] = ip_x(pop(1), row, xo, X0);
+    F[2] = ip_x(pop(2), row, xo, X2);
+    F[3] = ip_xi_y(pop(3), zb + i, yo, plane, Y0, XI + XI3 * 7);
+    F[4] = ip_xi_y(pop(4), zb + i, yo, plane, Y2, XI + XI4 * 7);
+    F[5] = ip_xi(pop(5), zb + (long long)j * nx + i, plane, XI + XI5 * 7);
+    F[6] = ip_xi(pop(6), zb + (long long)j * nx + i, plane, XI + XI6 * 7);
+    F[7] = ip_x_y_xi(pop(7), zb, xo, yo, plane, X0, Y0, XI + XI3 * 7);
+    F[8] = ip_x_y_xi(pop(8), zb, xo, yo, plane, X2, Y0, XI + XI3 * 7);
+    F[9] = ip_x_y_xi(pop(9), zb, xo, yo, plane, X0, Y2, XI + XI4 * 7);
+    F[10] = ip_x_y_xi(pop(10), zb, xo, yo, plane, X2, Y2, XI + XI4 * 7);
+    F[11] = ip_x_xi(pop(11), zb + (long long)j * nx, xo, plane, X0, XI + XI5 * 7);
+    F[12] = ip_x_xi(pop(12), zb + (long long)j * nx, xo, plane, X2, XI + XI5 * 7);
+    F[13] = ip_x_xi(pop(13), zb + (long long)j * nx, xo, plane, X0, XI + XI6 * 7);
+    F[14] = ip_x_xi(pop(14), zb + (long long)j * nx, xo, plane, X2, XI + XI6 * 7);
+    F[15] = ip_y_xi(pop(15), zb + i, yo, plane, Y0, XI + XI15 * 7);
+    F[16] = ip_y_xi(pop(16), zb + i, yo, plane, Y2, XI + XI16 * 7);
+    F[17] = ip_y_xi(pop(17), zb + i, yo, plane, Y0, XI + XI17 * 7);
+    F[18] = ip_y_xi(pop(18), zb + i, yo, plane, Y2, XI + XI18 * 7);
+
+    apply_walls<ST>(F, a, node, i, j, k, xo, yo, plane);
+
+    C tau_eff = (C)a.tau;
+    C rho, ux, uy, uz;
+    collide_node<CD::kDev>(F, (C)*a.rho_modify, (C)a.dt, (C)(*a.force), a.les ? a.t.les_delta[(long long)k * ny + j] : C(0),
+                           a.les, (C)a.tau, (C)a.cs, rho, ux, uy, uz, tau_eff);
+
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) CD::st(a.fn + (long long)q * N, node, F[q], a.scale[q]);
+    a.rho[node] = rho;
+    a.u[node] = ux;
+    a.v[node] = uy;
+    a.w[node] = uz;
+}
+
+// ------------------------------------------------------------------------------------
+// Bulk flux and forcing controller
+// ------------------------------------------------------------------------------------
+
+template <typename M>
+__global__ __launch_bounds__(256) void k_flux_probe(const M* __restrict__ v, const double* __restrict__ fw,
+                                                    double* acc, int nx, int ny, int nz, int jp)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nx * nz; t += blockDim.x) {
+        const int i = t % nx, k = t / nx;
+        s += (double)v[((long long)k * ny + jp) * nx + i] * fw[t];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += red[0];
+}
+
+hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* acc, int nx,
+                             int ny, int nz, int jplane)
+{
+    if (fp64)
+        hipLaunchKernelGGL(k_flux_probe<double>, dim3(1), dim3(256), 0, st, (const double*)v, fluxw, acc, nx, ny, nz,
+                           jplane);
+    else
+        hipLaunchKernelGGL(k_flux_probe<float>, dim3(1), dim3(256), 0, st, (const float*)v, fluxw, acc, nx, ny, nz,
+                           jplane);
+    return hipGetLastError();
+}
+
+__global__ void k_force_update(double* force, double* acc, double inv_area, int ndtfrc, double beta, double uref)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // Ub = sum / (LX (LZ - 1)) / NDTFRC;  F += beta (Uref - Ub) Uref / 3.036
+    const double ub = acc[0] * inv_area / ndtfrc;
+    acc[0] = 0.0;
+    force[0] = force[0] + beta * (uref - ub) * uref / 3.036;
+}
+
+hipError_t launch_force_update(hipStream_t st, double* force, double* acc, double inv_area, int ndtfrc, double beta,
+                               double uref)
+{
+    hipLaunchKernelGGL(k_force_update, dim3(1), dim3(64), 0, st, force, acc, inv_area, ndtfrc, beta, uref);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Dispatch
+// ------------------------------------------------------------------------------------
+
+template <Store ST>
+static hipError_t launch_pernode(hipStream_t st, const StepLaunch& L)
+{
+    StepArgs<ST> a = make_step_args<ST>(L);
+    const long long nb = (L.g.n + 255) / 256;
+    hipLaunchKernelGGL(k_collide_pernode<ST>, dim3((unsigned)nb), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiled(hipStream_t st, const StepLaunch& L);  // islbm_tiled.hip
+
+hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L)
+{
+    if (L.kernel == ISLBM_KERNEL_TILED) return launch_tiled(st, L);
+    switch (L.store) {
+    case Store::F64: return launch_pernode<Store::F64>(st, L);
+    case Store::F32: return launch_pernode<Store::F32>(st, L);
+    case Store::BF16: return launch_pernode<Store::BF16>(st, L);
+    case Store::FP16: return launch_pernode<Store::FP16>(st, L);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------------------------
+// Reference padded layout <-> unique nodes (FP64), for the reference-named entry points
+// ------------------------------------------------------------------------------------
+
+// padded (I,J,K) of unique (i,j,k) is (i+3, j+3, k+3); NX6 = nx+7, NY6 = ny+7, NZ6 = nz+6
+__global__ void k_padded_to_unique(PtrSet pad, PtrSet uni, int nx, int ny, int nz, int narr)
+{
+    const long long n = (long long)nx * ny * nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * narr) return;
+    const int q = (int)(t / n);
+    const long long id = t % n;
+    const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((long long)nx * ny));
+    const long long NX6 = nx + 7, NZ6 = nz + 6;
+    uni.p[q][id] = pad.p[q][(long long)(j + 3) * NX6 * NZ6 + (long long)(k + 3) * NX6 + (i + 3)];
+}
+
+// Every padded node of the interior, the periodic duplicates and the three x and y ghost
+// layers gets the unique node it is a periodic image of (what periodicSW and the halo
+// exchange leave behind, evolution.h:732-777, 980-994); z ghost layers are never read by
+// the step and are not written.
+__global__ void k_unique_to_padded(PtrSet uni, PtrSet pad, int nx, int ny, int nz, int narr)
+{
+    const long long NX6 = nx + 7, NY6 = ny + 7, NZ6 = nz + 6;
+    const long long per = NX6 * NY6 * nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * narr) return;
+    const int q = (int)(t / per);
+    const long long r = t % per;
+    const int I = (int)(r % NX6);
+    const int J = (int)((r / NX6) % NY6);
+    const int k = (int)(r / (NX6 * NY6));
+    const int i = wrap_index(I - 3, nx), j = wrap_index(J - 3, ny);
+    pad.p[q][(long long)J * NX6 * NZ6 + (long long)(k + 3) * NX6 + I] = uni.p[q][((long long)k * ny + j) * nx + i];
+}
+
+hipError_t launch_padded_to_unique(hipStream_t st, const PtrSet& pad, const PtrSet& uni, int nx, int ny, int nz,
+                                   int narr)
+{
+    const long long tot = (long long)nx * ny * nz * narr;
+    hipLaunchKernelGGL(k_padded_to_unique, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, pad, uni, nx, ny,
+                       nz, narr);
+    return hipGetLastError();
+}
+
+hipError_t launch_unique_to_padded(hipStream_t st, const PtrSet& uni, const PtrSet& pad, int nx, int ny, int nz,
+                                   int narr)
+{
+    const long long tot = (long long)(nx + 7) * (ny + 7) * nz * narr;
+    hipLaunchKernelGGL(k_unique_to_padded, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, uni, pad, nx, ny,
+                       nz, narr);
+    return hipGetLastError();
+}
+
+}  // namespace islbm

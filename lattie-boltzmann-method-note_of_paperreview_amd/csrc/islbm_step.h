@@ -1,0 +1,386 @@
+// islbm_step.h -- device building blocks of the stream-collide step: storage codecs,
+// 7-point interpolation in the reference's axis orders, wall treatments and the fused MRT
+// collision.  Included by the per-node and the tiled kernels.
+#pragma once
+
+#include "islbm_internal.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+namespace islbm {
+
+// ------------------------------------------------------------------------------------
+// Storage codecs.  kDev: the stored value is the deviation f - w_q (times scale_q).
+// ------------------------------------------------------------------------------------
+template <Store ST>
+struct Codec;
+
+template <>
+struct Codec<Store::F64> {
+    using raw = double;
+    using C = double;
+    static constexpr bool kDev = false;
+    __device__ static C ld(const raw* p, long long i, float) { return p[i]; }
+    __device__ static void st(raw* p, long long i, C v, float) { p[i] = v; }
+};
+
+template <>
+struct Codec<Store::F32> {
+    using raw = float;
+    using C = float;
+    static constexpr bool kDev = true;
+    __device__ static C ld(const raw* p, long long i, float) { return p[i]; }
+    __device__ static void st(raw* p, long long i, C v, float) { p[i] = v; }
+};
+
+template <>
+struct Codec<Store::BF16> {
+    using raw = uint16_t;
+    using C = float;
+    static constexpr bool kDev = true;
+    __device__ static C ld(const raw* p, long long i, float isc)
+    {
+        const uint32_t b = (uint32_t)p[i] << 16;
+        return __uint_as_float(b) * isc;
+    }
+    __device__ static void st(raw* p, long long i, C v, float sc)
+    {
+        // round to nearest even
+        const uint32_t b = __float_as_uint(v * sc);
+        const uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
+        p[i] = (uint16_t)(r >> 16);
+    }
+};
+
+template <>
+struct Codec<Store::FP16> {
+    using raw = _Float16;
+    using C = float;
+    static constexpr bool kDev = true;
+    __device__ static C ld(const raw* p, long long i, float isc) { return (float)p[i] * isc; }
+    __device__ static void st(raw* p, long long i, C v, float sc) { p[i] = (_Float16)(v * sc); }
+};
+
+template <Store ST>
+struct StepArgs {
+    using CD = Codec<ST>;
+    using raw = typename CD::raw;
+    using C = typename CD::C;
+    const raw* __restrict__ fo;
+    raw* __restrict__ fn;
+    C* __restrict__ rho;
+    C* __restrict__ u;
+    C* __restrict__ v;
+    C* __restrict__ w;
+    DevTables<C> t;
+    const double* force;
+    const double* rho_modify;
+    int nx, ny, nz;
+    long long n;
+    double tau, dt, cs;
+    int les;
+    float scale[NQ];
+    float inv_scale[NQ];
+};
+
+template <Store ST>
+inline StepArgs<ST> make_step_args(const StepLaunch& L)
+{
+    using A = StepArgs<ST>;
+    using raw = typename A::raw;
+    using C = typename A::C;
+    A a;
+    a.fo = (const raw*)L.fo;
+    a.fn = (raw*)L.fn;
+    a.rho = (C*)L.rho;
+    a.u = (C*)L.u;
+    a.v = (C*)L.v;
+    a.w = (C*)L.w;
+    a.t = *(const DevTables<C>*)L.tables;
+    a.force = L.force;
+    a.rho_modify = L.rho_modify;
+    a.nx = L.g.nx;
+    a.ny = L.g.ny;
+    a.nz = L.g.nz;
+    a.n = L.g.n;
+    a.tau = L.tau;
+    a.dt = L.dt;
+    a.cs = L.cs_smag;
+    a.les = L.cs_smag > 0.0;
+    for (int q = 0; q < NQ; ++q) {
+        a.scale[q] = L.scale ? L.scale[q] : 1.0f;
+        a.inv_scale[q] = L.scale ? 1.0f / L.scale[q] : 1.0f;
+    }
+    return a;
+}
+
+// Reads one population array, decoded to compute precision.
+template <Store ST>
+struct PopReader {
+    const typename Codec<ST>::raw* p;
+    float isc;
+    __device__ typename Codec<ST>::C operator()(long long i) const { return Codec<ST>::ld(p, i, isc); }
+};
+
+// ------------------------------------------------------------------------------------
+// 7-point Lagrange interpolation (interpolationHillISLBM.h).  Each helper keeps the
+// reference's inner-to-outer axis order; Intrpl7 sums left to right.
+//   xo[l]: wrapped x index of column i-3+l;  yo[n]: wrapped (j-3+n)*nx;  zb: z stencil base.
+// ------------------------------------------------------------------------------------
+
+// F1/F2_Intrpl7 (:14-20): x only, at this node's row
+template <class R, class C>
+__device__ inline C ip_x(const R& f, long long row, const int* xo, const C* wx)
+{
+    C r = f(row + xo[0]) * wx[0];
+#pragma unroll
+    for (int l = 1; l < 7; ++l) r = r + f(row + xo[l]) * wx[l];
+    return r;
+}
+
+// F5/F6_Intrpl7 (:60-66): xi only
+template <class R, class C>
+__device__ inline C ip_xi(const R& f, long long base, long long plane, const C* wz)
+{
+    C r = f(base) * wz[0];
+#pragma unroll
+    for (int m = 1; m < 7; ++m) r = r + f(base + m * plane) * wz[m];
+    return r;
+}
+
+// F3/F4_Intrpl7 (:22-58): xi inner, y outer.  base = zb + i.
+template <class R, class C>
+__device__ inline C ip_xi_y(const R& f, long long base, const int* yo, long long plane, const C* wy, const C* wz)
+{
+    C r = C(0);
+#pragma unroll
+    for (int n = 0; n < 7; ++n) {
+        C in = f(base + yo[n]) * wz[0];
+#pragma unroll
+        for (int m = 1; m < 7; ++m) in = in + f(base + m * plane + yo[n]) * wz[m];
+        r = (n == 0) ? in * wy[0] : r + in * wy[n];
+    }
+    return r;
+}
+
+// Y_XI_Intrpl7 (:150-161): y inner, xi outer.  base = zb + i.
+template <class R, class C>
+__device__ inline C ip_y_xi(const R& f, long long base, const int* yo, long long plane, const C* wy, const C* wz)
+{
+    C r = C(0);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+        C in = f(base + m * plane + yo[0]) * wy[0];
+#pragma unroll
+        for (int n = 1; n < 7; ++n) in = in + f(base + m * plane + yo[n]) * wy[n];
+        r = (m == 0) ? in * wz[0] : r + in * wz[m];
+    }
+    return r;
+}
+
+// X_XI_Intrpl7 (:137-148): x inner, xi outer.  base = zb + j*nx.
+template <class R, class C>
+__device__ inline C ip_x_xi(const R& f, long long base, const int* xo, long long plane, const C* wx, const C* wz)
+{
+    C r = C(0);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+        C in = f(base + m * plane + xo[0]) * wx[0];
+#pragma unroll
+        for (int l = 1; l < 7; ++l) in = in + f(base + m * plane + xo[l]) * wx[l];
+        r = (m == 0) ? in * wz[0] : r + in * wz[m];
+    }
+    return r;
+}
+
+// X_Y_XI_Intrpl7 (:68-135): x inner, y middle, xi outer.  base = zb.
+template <class R, class C>
+__device__ inline C ip_x_y_xi(const R& f, long long base, const int* xo, const int* yo, long long plane, const C* wx,
+                              const C* wy, const C* wz)
+{
+    C r = C(0);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+        C mid = C(0);
+#pragma unroll
+        for (int n = 0; n < 7; ++n) {
+            const long long b = base + m * plane + yo[n];
+            C in = f(b + xo[0]) * wx[0];
+#pragma unroll
+            for (int l = 1; l < 7; ++l) in = in + f(b + xo[l]) * wx[l];
+            mid = (n == 0) ? in * wy[0] : mid + in * wy[n];
+        }
+        r = (m == 0) ? mid * wz[0] : r + mid * wz[m];
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------------------------
+// Walls.  Flat-wall half-way bounce-back with the stored (previous post-collision)
+// populations of the node (evolution.h:482-496), then BFL on the first two layers above
+// the hill (evolution.h:505-572; weights read at layer*ny + j).  bflag holds the branch
+// chosen in FP64 on the host: 1 = linear (q > 1/2), 2 = interpolated (q < 1/2),
+// 3 = flagged with q == 1/2 (no change), 0 = not flagged.
+// ------------------------------------------------------------------------------------
+template <Store ST, class C>
+__device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long node, int i, int j, int k,
+                                   const int* xo, const int* yo, long long plane)
+{
+    const long long N = a.n;
+    auto at = [&](int q) { return Codec<ST>::ld(a.fo + (long long)q * N, node, a.inv_scale[q]); };
+    if (k == 0) {
+        F[5] = at(6); F[11] = at(14); F[12] = at(13); F[15] = at(18); F[16] = at(17);
+    }
+    if (k == a.nz - 1) {
+        F[6] = at(5); F[13] = at(12); F[14] = at(11); F[17] = at(16); F[18] = at(15);
+    }
+    if (k <= 1) {
+        const int nb = 2 * a.ny;
+        const int ib = k * a.ny + j;
+        auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * N, a.inv_scale[q]}; };
+        auto W = [&](int which) { return a.t.bw + ((long long)which * nb + ib) * 7; };
+        const int f3 = a.t.bflag[B3 * nb + ib], f4 = a.t.bflag[B4 * nb + ib];
+        const int f15 = a.t.bflag[B15 * nb + ib], f16 = a.t.bflag[B16 * nb + ib];
+        if (f3 == 1) {
+            const C q = a.t.bq[B3 * nb + ib];
+            const C c1 = C(1) / (C(2) * q), c2 = (C(2) * q - C(1)) / (C(2) * q);
+            F[3] = c1 * at(4) + c2 * at(3);
+            F[7] = c1 * at(10) + c2 * at(7);
+            F[8] = c1 * at(9) + c2 * at(8);
+        } else if (f3 == 2) {
+            F[3] = ip_y_xi(pop(4), (long long)i, yo, plane, W(BY378), W(BXI378));
+            F[7] = ip_x_y_xi(pop(10), 0LL, xo, yo, plane, W(BX37), W(BY378), W(BXI378));
+            F[8] = ip_x_y_xi(pop(9), 0LL, xo, yo, plane, W(BX38), W(BY378), W(BXI378));
+        }
+        if (f4 == 1) {
+            const C q = a.t.bq[B4 * nb + ib];
+            const C c1 = C(1) / (C(2) * q), c2 = (C(2) * q - C(1)) / (C(2) * q);
+            F[4] = c1 * at(3) + c2 * at(4);
+            F[9] = c1 * at(8) + c2 * at(9);
+            F[10] = c1 * at(7) + c2 * at(10);
+        } else if (f4 == 2) {
+            F[4] = ip_y_xi(pop(3), (long long)i, yo, plane, W(BY4910), W(BXI4910));
+            F[9] = ip_x_y_xi(pop(8), 0LL, xo, yo, plane, W(BX49), W(BY4910), W(BXI4910));
+            F[10] = ip_x_y_xi(pop(7), 0LL, xo, yo, plane, W(BX410), W(BY4910), W(BXI4910));
+        }
+        if (f15 == 1) {
+            const C q = a.t.bq[B15 * nb + ib];
+            F[15] = (C(1) / (C(2) * q)) * at(18) + ((C(2) * q - C(1)) / (C(2) * q)) * at(15);
+        } else if (f15 == 2) {
+            F[15] = ip_y_xi(pop(18), (long long)i, yo, plane, W(BY15), W(BXI15));
+        }
+        if (f16 == 1) {
+            const C q = a.t.bq[B16 * nb + ib];
+            F[16] = (C(1) / (C(2) * q)) * at(17) + ((C(2) * q - C(1)) / (C(2) * q)) * at(16);
+        } else if (f16 == 2) {
+            F[16] = ip_y_xi(pop(17), (long long)i, yo, plane, W(BY16), W(BXI16));
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Moments, equilibrium and MRT collision with body force (evolution.h:597-632,
+// MRT_Process.h:4-179).  With orthogonal M:  f' = f - M^T diag(s_a/|M_a|^2) M (f - f_eq),
+// evaluated with the integer rows of M; conserved rows (s = 0) are skipped.
+// Body force on the y-moving populations: +-F dt/6 (3, 4), +-F dt/12 (7-10, 15-18).
+// kDev: F holds f - w_q and rho is rebuilt as 1 + sum.
+// LES: tau* = tau + 3 nu_t/dt, nu_t = (Cs D)^2 |S|, |S| from Q = Pi_neq:Pi_neq in closed form.
+// ------------------------------------------------------------------------------------
+template <bool kDev, class C>
+__device__ inline void collide_node(C (&F)[NQ], C rho_modify, C dt, C force, C delta, int les, C tau, C cs, C& rho_out,
+                                    C& ux_out, C& uy_out, C& uz_out, C& tau_eff)
+{
+    F[0] = F[0] + rho_modify;
+    C s = F[0];
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) s = s + F[q];
+    const C rho = kDev ? C(1) + s : s;
+    const C ux = (F[1] + F[7] + F[9] + F[11] + F[13] - (F[2] + F[8] + F[10] + F[12] + F[14])) / rho;
+    const C uy = (F[3] + F[7] + F[8] + F[15] + F[17] - (F[4] + F[9] + F[10] + F[16] + F[18])) / rho;
+    const C uz = (F[5] + F[11] + F[12] + F[15] + F[16] - (F[6] + F[13] + F[14] + F[17] + F[18])) / rho;
+    const C udot = ux * ux + uy * uy + uz * uz;
+
+    C d[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        C eu = C(0);
+        if (cx(q)) eu = eu + C(cx(q)) * ux;
+        if (cy(q)) eu = eu + C(cy(q)) * uy;
+        if (cz(q)) eu = eu + C(cz(q)) * uz;
+        const C p = C(3.0) * eu + C(4.5) * eu * eu - C(1.5) * udot;
+        const C feq = kDev ? C(wq(q)) * (s + rho * p) : C(wq(q)) * rho * (C(1.0) + p);
+        d[q] = F[q] - feq;
+    }
+
+    C s_nu = C(1) / tau;
+    tau_eff = tau;
+    if (les) {
+        C pxx = 0, pyy = 0, pzz = 0, pxy = 0, pyz = 0, pxz = 0;
+#pragma unroll
+        for (int q = 1; q < NQ; ++q) {
+            if (cx(q)) pxx += d[q];
+            if (cy(q)) pyy += d[q];
+            if (cz(q)) pzz += d[q];
+            if (cx(q) * cy(q)) pxy += C(cx(q) * cy(q)) * d[q];
+            if (cy(q) * cz(q)) pyz += C(cy(q) * cz(q)) * d[q];
+            if (cx(q) * cz(q)) pxz += C(cx(q) * cz(q)) * d[q];
+        }
+        const C Q = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (pxy * pxy + pyz * pyz + pxz * pxz);
+        const C Cd = cs * cs * delta * delta;
+        const C b = tau * dt;
+        const C S = (sqrt(b * b + C(18.0 * 1.4142135623730951) * Cd * sqrt(Q) / rho) - b) / (C(6) * Cd);
+        tau_eff = tau + C(3) * Cd * S / dt;
+        s_nu = C(1) / tau_eff;
+    }
+
+    C r[NQ];
+#pragma unroll
+    for (int a = 0; a < NQ; ++a) {
+        if (is_conserved(a)) {
+            r[a] = C(0);
+            continue;
+        }
+        C acc = C(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int m = mrow(a, q);
+            if (m == 1)
+                acc = acc + d[q];
+            else if (m == -1)
+                acc = acc - d[q];
+            else if (m != 0)
+                acc = acc + C(m) * d[q];
+        }
+        const C rate = is_stress(a) ? s_nu : C(fixed_rate(a));
+        r[a] = acc * (rate / C(mnorm(a)));
+    }
+    const C fdt = force * dt;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        C acc = C(0);
+#pragma unroll
+        for (int a = 0; a < NQ; ++a) {
+            if (is_conserved(a)) continue;
+            const int m = mrow(a, q);
+            if (m == 1)
+                acc = acc + r[a];
+            else if (m == -1)
+                acc = acc - r[a];
+            else if (m != 0)
+                acc = acc + C(m) * r[a];
+        }
+        C v = F[q] - acc;
+        if (cy(q) != 0) {
+            const C c = (cx(q) == 0 && cz(q) == 0) ? C(1.0 / 6) : C(1.0 / 12);
+            v = (cy(q) > 0) ? v + c * fdt : v - c * fdt;
+        }
+        F[q] = v;
+    }
+    rho_out = rho;
+    ux_out = ux;
+    uy_out = uy;
+    uz_out = uz;
+}
+
+}  // namespace islbm

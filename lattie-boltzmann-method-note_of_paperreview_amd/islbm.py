@@ -1,0 +1,251 @@
+"""ctypes binding of the gfx950 ISLBM solver library (include/islbm.h).
+
+The library is loaded from this package's lib/ folder; a missing library is an error,
+never a silent fallback.  Arrays are numpy float64 in unique-node layout:
+populations (19, nz, ny, nx), macroscopic fields (nz, ny, nx), with the reference's axes
+(x spanwise, y streamwise, z wall-normal).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libislbm_amd.so")
+
+FP64, FP32, BF16, FP16 = 0, 1, 2, 3
+MESH_BISECT, MESH_STRETCH = 0, 1
+KERNEL_AUTO, KERNEL_PERNODE, KERNEL_TILED = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "fp16": FP16}
+
+HEADER_SYMBOLS = [
+    "islbm_default_params", "islbm_required_bytes", "islbm_create", "islbm_destroy",
+    "islbm_init_equilibrium", "islbm_upload_f", "islbm_download_f", "islbm_download_macros",
+    "islbm_step", "islbm_sync", "islbm_modify_forcing", "islbm_get_force", "islbm_set_force",
+    "islbm_set_rho_modify", "islbm_get_flux_sum", "islbm_scalars", "islbm_step_count",
+    "islbm_kernel_in_use", "islbm_get_table", "islbm_time_steps", "islbm_last_error",
+    "islbm_ref_init", "islbm_ref_shutdown", "islbm_ref_device_arrays",
+    "Launch_CollisionStreaming", "Launch_ModifyForcingTerm",
+]
+
+
+class Params(ctypes.Structure):
+    """Run-time form of the reference's compile-time parameter set (variables.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("lxi", ctypes.c_double), ("hill_alpha", ctypes.c_double),
+        ("flat", ctypes.c_int32), ("mesh_mode", ctypes.c_int32), ("stretch", ctypes.c_double),
+        ("tau", ctypes.c_double), ("re", ctypes.c_double), ("u_ref", ctypes.c_double),
+        ("cs_smag", ctypes.c_double), ("ndtfrc", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("kernel", ctypes.c_int32), ("device", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ISLBMError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the solver library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ISLBMError(f"solver library missing: {LIB_PATH} (run build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, i32 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32
+    pp = ctypes.POINTER(Params)
+    L.islbm_default_params.argtypes = [pp]
+    L.islbm_required_bytes.argtypes = [pp, ctypes.POINTER(ctypes.c_size_t)]
+    L.islbm_create.argtypes = [pp, ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.islbm_destroy.argtypes = [vp]
+    L.islbm_init_equilibrium.argtypes = [vp, ctypes.c_double, dp, ctypes.c_double, ctypes.c_uint64]
+    L.islbm_upload_f.argtypes = [vp, dp]
+    L.islbm_download_f.argtypes = [vp, dp]
+    L.islbm_download_macros.argtypes = [vp, dp, dp, dp, dp]
+    L.islbm_step.argtypes = [vp, i32]
+    L.islbm_sync.argtypes = [vp]
+    L.islbm_modify_forcing.argtypes = [vp]
+    L.islbm_get_force.argtypes = [vp, dp]
+    L.islbm_set_force.argtypes = [vp, ctypes.c_double]
+    L.islbm_set_rho_modify.argtypes = [vp, ctypes.c_double]
+    L.islbm_get_flux_sum.argtypes = [vp, dp]
+    L.islbm_scalars.argtypes = [vp, dp]
+    L.islbm_step_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.islbm_kernel_in_use.argtypes = [vp, ctypes.POINTER(i32)]
+    L.islbm_get_table.argtypes = [vp, ctypes.c_char_p, dp, i32]
+    L.islbm_time_steps.argtypes = [vp, i32, dp]
+    L.islbm_last_error.restype = ctypes.c_char_p
+    L.islbm_ref_init.argtypes = [pp]
+    L.islbm_ref_device_arrays.argtypes = [ctypes.POINTER(vp)] * 6
+    L.Launch_CollisionStreaming.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.Launch_CollisionStreaming.restype = None
+    L.Launch_ModifyForcingTerm.argtypes = []
+    L.Launch_ModifyForcingTerm.restype = None
+    for name in HEADER_SYMBOLS:
+        fn = getattr(L, name)
+        if fn.restype is ctypes.c_int and name not in ("islbm_last_error",):
+            fn.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = lib().islbm_last_error().decode(errors="replace")
+        raise ISLBMError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_params(**kw) -> Params:
+    """The reference's shipped parameter set (variables.h), updated by keyword overrides."""
+    p = Params()
+    _check(lib().islbm_default_params(ctypes.byref(p)), "islbm_default_params")
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        setattr(p, k, v)
+    return p
+
+
+def required_bytes(p: Params) -> int:
+    b = ctypes.c_size_t()
+    _check(lib().islbm_required_bytes(ctypes.byref(p), ctypes.byref(b)), "islbm_required_bytes")
+    return b.value
+
+
+class Solver:
+    """One device-resident periodic-hill solve."""
+
+    def __init__(self, params: Params, hbm_budget: int = 0):
+        self.p = params
+        h = ctypes.c_void_p()
+        _check(lib().islbm_create(ctypes.byref(params), int(hbm_budget), ctypes.byref(h)), "islbm_create")
+        self.h = h
+        self.shape = (params.nz, params.ny, params.nx)
+        self.n = params.nx * params.ny * params.nz
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().islbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state -------------------------------------------------------------------------
+    def init_equilibrium(self, rho0=1.0, u0=(0.0, 0.0, 0.0), amp=0.0, seed=20261203):
+        u = np.ascontiguousarray(u0, dtype=np.float64)
+        _check(lib().islbm_init_equilibrium(self.h, float(rho0), _dp(u), float(amp), int(seed)), "init")
+
+    def upload_f(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        if f.size != 19 * self.n:
+            raise ValueError("population array must hold 19 * nx * ny * nz values")
+        _check(lib().islbm_upload_f(self.h, _dp(f)), "islbm_upload_f")
+
+    def f(self) -> np.ndarray:
+        out = np.empty((19,) + self.shape)
+        _check(lib().islbm_download_f(self.h, _dp(out)), "islbm_download_f")
+        return out
+
+    def macros(self):
+        r, u, v, w = (np.empty(self.shape) for _ in range(4))
+        _check(lib().islbm_download_macros(self.h, _dp(r), _dp(u), _dp(v), _dp(w)), "islbm_download_macros")
+        return r, u, v, w
+
+    # -- stepping ------------------------------------------------------------------------
+    def step(self, n: int = 1, sync: bool = True):
+        _check(lib().islbm_step(self.h, int(n)), "islbm_step")
+        if sync:
+            self.sync()
+
+    def sync(self):
+        _check(lib().islbm_sync(self.h), "islbm_sync")
+
+    def modify_forcing(self):
+        _check(lib().islbm_modify_forcing(self.h), "islbm_modify_forcing")
+
+    @property
+    def force(self) -> float:
+        v = ctypes.c_double()
+        _check(lib().islbm_get_force(self.h, ctypes.byref(v)), "islbm_get_force")
+        return v.value
+
+    @force.setter
+    def force(self, val: float):
+        _check(lib().islbm_set_force(self.h, float(val)), "islbm_set_force")
+
+    def set_rho_modify(self, v: float):
+        _check(lib().islbm_set_rho_modify(self.h, float(v)), "islbm_set_rho_modify")
+
+    def flux_sum(self) -> float:
+        v = ctypes.c_double()
+        _check(lib().islbm_get_flux_sum(self.h, ctypes.byref(v)), "islbm_get_flux_sum")
+        return v.value
+
+    def scalars(self) -> dict:
+        o = np.zeros(8)
+        _check(lib().islbm_scalars(self.h, _dp(o)), "islbm_scalars")
+        return dict(zip(["dt", "tau", "nu", "uref", "a", "dx", "dy", "force0"], o.tolist()))
+
+    def step_count(self) -> int:
+        v = ctypes.c_int64()
+        _check(lib().islbm_step_count(self.h, ctypes.byref(v)), "islbm_step_count")
+        return v.value
+
+    def kernel_in_use(self) -> int:
+        v = ctypes.c_int32()
+        _check(lib().islbm_kernel_in_use(self.h, ctypes.byref(v)), "islbm_kernel_in_use")
+        return v.value
+
+    def table(self, name: str) -> np.ndarray:
+        cap = 16
+        while True:
+            out = np.zeros(cap)
+            n = lib().islbm_get_table(self.h, name.encode(), _dp(out), cap)
+            if n >= 0:
+                return out[:n].copy()
+            if n == -1 or -n <= cap:
+                _check(n, f"islbm_get_table({name})")
+            cap = -n
+
+    def time_steps(self, n: int):
+        """(ms for n steps, summed ms of the stream-collide launches) from HIP events."""
+        o = np.zeros(2)
+        _check(lib().islbm_time_steps(self.h, int(n), _dp(o)), "islbm_time_steps")
+        return float(o[0]), float(o[1])
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    den = float(np.linalg.norm(b.ravel()))
+    return float(np.linalg.norm((a - b).ravel())) / (den if den > 0 else 1.0)
+
+
+def rel_l2_vec(us, vs) -> float:
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(us, vs))
+    den = sum(float(np.sum(b ** 2)) for b in vs)
+    return (num ** 0.5) / (den ** 0.5 if den > 0 else 1.0)

@@ -1,0 +1,918 @@
+/*
+ * islbm_oracle.c -- FP64 CPU restatement of the reference ISLBM periodic-hill step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the CPU baseline
+ * timed beside it.  Nothing in the product links or calls this file.
+ *
+ * Every routine cites the reference file:line it restates.  Reference paths are
+ * relative to the reference's code folder (".../0.可更改的code/").  Quirk
+ * resolutions (the reference cannot be run as shipped):
+ *   Q1  the full 19-moment transform of MRT_Process.h:4-62 is used (MRT_Matrix.h:74-78
+ *       shadows it with an m0-only macro).
+ *   Q3  BFL q<0.5 weights are read at (k-3)*NYD6+j, where initialization.h:286-328
+ *       stores them (evolution.h:525-527,544-546,557,568 pass i and j instead).
+ *   Q7  the bulk-flux accumulator samples the periodic image plane J=0 (y = LY-3dy)
+ *       holding the previous step's v, as AccumulateUbulk does (evolution.h:826-840,936).
+ *   Q8  rho_modify defaults to 0 (its computation lives in the absent main.cu).
+ *   Bisection loops (initializationTool.h:22-32, 96-103, 135-147) are capped at 400
+ *   iterations; the reference would not terminate in the capped cases.
+ */
+#include "islbm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NQ 19
+#define BISECT_CAP 400
+
+/* D3Q19 velocity set and weights: initialization.h:7-12. */
+static const int E[NQ][3] = {
+    {0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},  {0, 0, -1},
+    {1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1}, {1, 0, -1},
+    {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+static const double W[NQ] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                             1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                             1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                             1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+/* ---------------------------------------------------------------------------------- */
+/* Geometry                                                                            */
+/* ---------------------------------------------------------------------------------- */
+
+/* Reference hill profile H(y): model.h:4-68 (ERCOFTAC piecewise cubic in units of h=28mm,
+ * wrapped periodically in y).  hill_alpha scales the flank length (alpha = 1 is the
+ * reference expression evaluated operation for operation).  flat: ChannelFunction
+ * (model.h:70-73). */
+double or_hill(double Y, double ly, double alpha, int flat)
+{
+    if (flat) return 0.0;
+    double Yb;
+    double model = 0.0;
+    if (Y < 0.0)
+        Yb = Y + ly;
+    else if (Y > ly)
+        Yb = Y - ly;
+    else
+        Yb = Y;
+    /* breakpoints (units of h) scaled by alpha; alpha = 1 gives the reference constants */
+    const double s9 = (54. / 28.) * (9. / 54.) * alpha, s14 = (54. / 28.) * (14. / 54.) * alpha;
+    const double s20 = (54. / 28.) * (20. / 54.) * alpha, s30 = (54. / 28.) * (30. / 54.) * alpha;
+    const double s40 = (54. / 28.) * (40. / 54.) * alpha, s54 = (54. / 28.) * (54. / 54.) * alpha;
+    const double yl = (alpha == 1.0) ? Yb : Yb / alpha;               /* left-flank argument */
+    const double yr = (alpha == 1.0) ? (ly - Yb) : (ly - Yb) / alpha; /* right-flank argument */
+    /* left flank: model.h:17-34 */
+    if (Yb <= s9) {
+        double c = 28. + 0.006775070969851 * yl * 28 * yl * 28 - 0.0021245277758000 * yl * 28 * yl * 28 * yl * 28;
+        model = 1. / 28. * (28. < c ? 28. : c);
+    }
+    if (Yb > s9 && Yb <= s14)
+        model = 1. / 28. * (25.07355893131 + 0.9754803562315 * yl * 28 - 0.1016116352781 * yl * 28 * yl * 28 + 0.001889794677828 * yl * 28 * yl * 28 * yl * 28);
+    if (Yb > s14 && Yb <= s20)
+        model = 1. / 28. * (25.79601052357 + 0.8206693007457 * yl * 28 - 0.09055370274339 * yl * 28 * yl * 28 + 0.001626510569859 * yl * 28 * yl * 28 * yl * 28);
+    if (Yb > s20 && Yb <= s30)
+        model = 1. / 28. * (40.46435022819 - 1.379581654948 * yl * 28 + 0.019458845041284 * yl * 28 * yl * 28 - 0.0002070318932190 * yl * 28 * yl * 28 * yl * 28);
+    if (Yb > s30 && Yb <= s40)
+        model = 1. / 28. * (17.92461334664 + 0.8743920332081 * yl * 28 - 0.05567361123058 * yl * 28 * yl * 28 + 0.0006277731764683 * yl * 28 * yl * 28 * yl * 28);
+    if (Yb > s40 && Yb <= s54) {
+        double c = 56.39011190988 - 2.010520359035 * yl * 28 + 0.01644919857549 * yl * 28 * yl * 28 + 0.00002674976141766 * yl * 28 * yl * 28 * yl * 28;
+        model = 1. / 28. * (0. > c ? 0. : c);
+    }
+    /* right flank: model.h:36-53, the same polynomials in (LY - y), thresholds in y */
+    if (Yb < ly - s40 && Yb >= ly - s54) {
+        double c = 56.39011190988 - 2.010520359035 * yr * 28 + 0.01644919857549 * yr * 28 * yr * 28 + 0.00002674976141766 * yr * 28 * yr * 28 * yr * 28;
+        model = 1. / 28. * (0. > c ? 0. : c);
+    }
+    if (Yb < ly - s30 && Yb >= ly - s40)
+        model = 1. / 28. * (17.92461334664 + 0.8743920332081 * yr * 28 - 0.05567361123058 * yr * 28 * yr * 28 + 0.0006277731764683 * yr * 28 * yr * 28 * yr * 28);
+    if (Yb < ly - s20 && Yb >= ly - s30)
+        model = 1. / 28. * (40.46435022819 - 1.379581654948 * yr * 28 + 0.019458845041284 * yr * 28 * yr * 28 - 0.0002070318932190 * yr * 28 * yr * 28 * yr * 28);
+    if (Yb < ly - s14 && Yb >= ly - s20)
+        model = 1. / 28. * (25.79601052357 + 0.8206693007457 * yr * 28 - 0.09055370274339 * yr * 28 * yr * 28 + 0.001626510569859 * yr * 28 * yr * 28 * yr * 28);
+    if (Yb < ly - s9 && Yb >= ly - s14)
+        model = 1. / 28. * (25.07355893131 + 0.9754803562315 * yr * 28 - 0.1016116352781 * yr * 28 * yr * 28 + 0.001889794677828 * yr * 28 * yr * 28 * yr * 28);
+    if (Yb >= ly - s9) {
+        double c = 28. + 0.006775070969851 * yr * 28 * yr * 28 - 0.0021245277758000 * yr * 28 * yr * 28 * yr * 28;
+        model = 1. / 28. * (28. < c ? 28. : c);
+    }
+    return model;
+}
+
+/* tanhFunction: initializationTool.h:4-7. */
+double or_tanh_map(double L, double lattice, double a, int j, int n)
+{
+    return L / 2.0 + lattice / 2.0 +
+           ((L / 2.0) / a) * tanh((-1.0 + 2.0 * (double)(j) / (double)(n)) / 2.0 * log((1.0 + a) / (1.0 - a)));
+}
+
+/* GetNonuniParameter: initializationTool.h:16-39 (first spacing == minSize). */
+double or_bisect_a(double lz, double minsize, int nz, double hill0)
+{
+    const double total = lz - hill0 - minsize;
+    double lo = 0.1, hi = 1.0, mid = 0.0, d;
+    int it = 0;
+    do {
+        mid = (lo + hi) / 2.0;
+        double x0 = or_tanh_map(total, minsize, mid, 0, nz - 1);
+        double x1 = or_tanh_map(total, minsize, mid, 1, nz - 1);
+        d = x1 - x0;
+        if (d - minsize >= 0.0)
+            lo = mid;
+        else
+            hi = mid;
+    } while (fabs(d - minsize) > 1e-14 && ++it < BISECT_CAP);
+    return mid;
+}
+
+/* Lagrange_6th: initializationTool.h:60-66; node m of the 7 is the pivot, the others are
+ * multiplied in increasing index order as GetParameter_6th (initializationTool.h:68-79)
+ * passes them. */
+double or_lagrange6(double x, const double* p, int m)
+{
+    double xi = p[m];
+    double r = 1.0;
+    int first = 1;
+    for (int n = 0; n < 7; ++n) {
+        if (n == m) continue;
+        if (first) {
+            r = (x - p[n]) / (xi - p[n]);
+            first = 0;
+        } else {
+            r = r * (x - p[n]) / (xi - p[n]);
+        }
+    }
+    return r;
+}
+
+/* GetParameter_6th: initializationTool.h:68-79.  out[m] weight of node pos[n+m]. */
+static void lagrange7(double* out7, double x, const double* pos, int n)
+{
+    for (int m = 0; m < 7; ++m) out7[m] = or_lagrange6(x, pos + n, m);
+}
+
+/* GetDeltaHorizontal: initializationTool.h:90-107. */
+double or_delta_horizontal(double z_target, double y_large, double y_small, double y_point,
+                           double ly, double alpha, double minsize)
+{
+    double t0 = y_large, t1 = y_small, mid;
+    int it = 0;
+    do {
+        mid = (t0 + t1) / 2.0;
+        if (or_hill(mid, ly, alpha, 0) >= z_target)
+            t0 = mid;
+        else
+            t1 = mid;
+    } while (fabs(or_hill(mid, ly, alpha, 0) - z_target) > 1e-13 && ++it < BISECT_CAP);
+    return minsize - 2.0 * fabs(y_point - mid);
+}
+
+/* GetDelta45Degree: initializationTool.h:125-154 (the fabs of a truth value, Q4, does not
+ * change the branch and is dropped). */
+double or_delta_45(double z_pnt, double y_pnt, double y_zdom, double y_ydom,
+                   double ly, double alpha, double minsize)
+{
+    double t0 = y_zdom, t1 = y_ydom, mid;
+    int it = 0;
+    do {
+        mid = (t0 + t1) / 2.0;
+        if (fabs(or_hill(mid, ly, alpha, 0) - z_pnt) > fabs(mid - y_pnt))
+            t0 = mid;
+        else
+            t1 = mid;
+    } while (fabs(fabs(or_hill(mid, ly, alpha, 0) - z_pnt) - fabs(mid - y_pnt)) > 1e-13 &&
+             ++it < BISECT_CAP);
+    return minsize - 2.0 * fabs(y_pnt - mid);
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* State                                                                               */
+/* ---------------------------------------------------------------------------------- */
+
+enum { XI3, XI4, XI5, XI6, XI15, XI16, XI17, XI18, NXI };
+enum { B3, B4, B15, B16, NB };
+enum { BX37, BX38, BY378, BXI378, BX49, BX410, BY4910, BXI4910, BY15, BXI15, BY16, BXI16, NBW };
+
+struct or_state {
+    or_params p;
+    int NX6, NY6, NZ6;
+    long nface, ntot;
+    double minsize, dt, tau, nu, uref, a, dx, dy;
+    double *x, *y, *z, *xi;               /* x[NX6] y[NY6] z[NY6*NZ6] xi[NZ6] */
+    double *xp0, *xp2;                    /* [7][NX6] */
+    double *yp0, *yp2;                    /* [7][NY6] */
+    double *xiw[NXI];                     /* [7][NY6*NZ6] */
+    int *bflag[NB];                       /* [2*NY6] */
+    double *bq[NB];                       /* [2*NY6] */
+    double *bw[NBW];                      /* [7][2*NY6] */
+    double *f[2][NQ];
+    double *rho, *u, *v, *w;
+    double *ub;                           /* [NZ6*NX6] */
+    double force, rho_modify;
+    long step;
+    int cur;
+    double M[NQ][NQ], Mi[NQ][NQ];
+};
+
+/* D3Q19 d'Humieres moment basis (MRT_Matrix.h:27-46, Krueger p.672), generated from its
+ * polynomial definitions; Mi = M^T diag(M M^T)^-1 (the rows are orthogonal), which is the
+ * table of MRT_Matrix.h:49-68. */
+static void build_mrt(double M[NQ][NQ], double Mi[NQ][NQ])
+{
+    for (int i = 0; i < NQ; ++i) {
+        const double ex = E[i][0], ey = E[i][1], ez = E[i][2];
+        const double c2 = ex * ex + ey * ey + ez * ez;
+        const double r[NQ] = {
+            1.0,
+            19.0 * c2 - 30.0,
+            (21.0 * c2 * c2 - 53.0 * c2 + 24.0) / 2.0,
+            ex, (5.0 * c2 - 9.0) * ex,
+            ey, (5.0 * c2 - 9.0) * ey,
+            ez, (5.0 * c2 - 9.0) * ez,
+            3.0 * ex * ex - c2, (3.0 * c2 - 5.0) * (3.0 * ex * ex - c2),
+            ey * ey - ez * ez, (3.0 * c2 - 5.0) * (ey * ey - ez * ez),
+            ex * ey, ey * ez, ex * ez,
+            (ey * ey - ez * ez) * ex, (ez * ez - ex * ex) * ey, (ex * ex - ey * ey) * ez};
+        for (int a = 0; a < NQ; ++a) M[a][i] = r[a];
+    }
+    for (int a = 0; a < NQ; ++a) {
+        double n2 = 0.0;
+        for (int i = 0; i < NQ; ++i) n2 += M[a][i] * M[a][i];
+        for (int i = 0; i < NQ; ++i) Mi[i][a] = M[a][i] / n2;
+    }
+}
+
+void or_mrt_matrices(double* Mo, double* Mio)
+{
+    double M[NQ][NQ], Mi[NQ][NQ];
+    build_mrt(M, Mi);
+    memcpy(Mo, M, sizeof M);
+    memcpy(Mio, Mi, sizeof Mi);
+}
+
+static double* zalloc(long n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+
+/* GetXiParameter: initialization.h:185-201.  Stencil start 3 near the bottom (k<=6),
+ * NZ6-10 near the top (k>=NZ6-7), else k-3 (padded k). */
+static void xi_weights(const or_state* s, double* out7, double pos_z, double pos_y, int K)
+{
+    const or_params* p = &s->p;
+    double H = or_hill(pos_y, p->ly, p->hill_alpha, p->flat);
+    double L = p->lz - H - s->minsize;
+    double pos_xi = p->lxi * (pos_z - (H + s->minsize / 2.0)) / L;
+    int n;
+    if (K >= 3 && K <= 6)
+        n = 3;
+    else if (K >= s->NZ6 - 7 && K <= s->NZ6 - 4)
+        n = s->NZ6 - 10;
+    else
+        n = K - 3;
+    lagrange7(out7, pos_xi, s->xi, n);
+}
+
+static void set7(double* base, long stride, long idx, const double* w7)
+{
+    for (int m = 0; m < 7; ++m) base[m * stride + idx] = w7[m];
+}
+
+or_state* or_create(const or_params* pin)
+{
+    or_state* s = (or_state*)calloc(1, sizeof(or_state));
+    s->p = *pin;
+    const or_params* p = &s->p;
+    const int NX6 = p->nx + 7, NY6 = p->ny + 7, NZ6 = p->nz + 6;
+    s->NX6 = NX6; s->NY6 = NY6; s->NZ6 = NZ6;
+    s->nface = (long)NX6 * NZ6;
+    s->ntot = s->nface * NY6;
+
+    /* variables.h:23,76,95-99 */
+    s->minsize = (p->lz - 1.0) / (NZ6 - 6) * p->cfl;
+    s->dt = s->minsize;
+    if (p->tau > 0.0) {
+        s->tau = p->tau;
+        s->nu = (s->tau - 0.5) / 3.0 * s->dt;
+        s->uref = p->re * s->nu;
+    } else {
+        s->nu = p->u_ref / p->re;
+        s->tau = 3.0 * s->nu / s->dt + 0.5;
+        s->uref = p->re * s->nu;
+    }
+    build_mrt(s->M, s->Mi);
+
+    /* GenerateMesh_X/Y: initialization.h:51-108 (jp = 1). */
+    s->x = zalloc(NX6); s->y = zalloc(NY6); s->z = zalloc((long)NY6 * NZ6); s->xi = zalloc(NZ6);
+    s->dx = p->lx / (double)(NX6 - 2 * 3 - 1);
+    s->dy = p->ly / (double)(NY6 - 2 * 3 - 1);
+    for (int i = 0; i < NX6; ++i) s->x[i] = s->dx * ((double)(i - 3));
+    for (int j = 0; j < NY6; ++j) s->y[j] = s->dy * ((double)(j - 3));
+
+    /* GenerateMesh_Z: initialization.h:110-137. */
+    const double hill0 = or_hill(0.0, p->ly, p->hill_alpha, p->flat);
+    if (p->mesh_mode == 0)
+        s->a = or_bisect_a(p->lz, s->minsize, p->nz, hill0);
+    else
+        s->a = tanh(p->stretch / 2.0);
+    for (int j = 0; j < NY6; ++j) {
+        double H = or_hill(s->y[j], p->ly, p->hill_alpha, p->flat);
+        double total = p->lz - H - s->minsize;
+        for (int k = 3; k < NZ6 - 3; ++k)
+            s->z[j * NZ6 + k] = or_tanh_map(total, s->minsize, s->a, k - 3, NZ6 - 7) + H;
+        s->z[j * NZ6 + 2] = H;
+        s->z[j * NZ6 + (NZ6 - 3)] = p->lz;
+    }
+    for (int k = 3; k < NZ6 - 3; ++k) s->xi[k] = or_tanh_map(p->lxi, s->minsize, s->a, k - 3, NZ6 - 7) - s->minsize / 2.0;
+
+    /* GetIntrplParameter_X/Y: initialization.h:203-229. */
+    s->xp0 = zalloc(7L * NX6); s->xp2 = zalloc(7L * NX6);
+    s->yp0 = zalloc(7L * NY6); s->yp2 = zalloc(7L * NY6);
+    double w7[7];
+    for (int i = 3; i < NX6 - 3; ++i) {
+        lagrange7(w7, s->x[i] - s->minsize, s->x, i - 3); set7(s->xp0, NX6, i, w7);
+        lagrange7(w7, s->x[i] + s->minsize, s->x, i - 3); set7(s->xp2, NX6, i, w7);
+    }
+    for (int j = 3; j < NY6 - 3; ++j) {
+        lagrange7(w7, s->y[j] - s->minsize, s->y, j - 3); set7(s->yp0, NY6, j, w7);
+        lagrange7(w7, s->y[j] + s->minsize, s->y, j - 3); set7(s->yp2, NY6, j, w7);
+    }
+
+    /* GetIntrplParameter_Xi: initialization.h:231-258. */
+    const long nyz = (long)NY6 * NZ6;
+    for (int f = 0; f < NXI; ++f) s->xiw[f] = zalloc(7 * nyz);
+    const double ms = s->minsize;
+    for (int j = 3; j < NY6 - 3; ++j) {
+        for (int k = 3; k < NZ6 - 3; ++k) {
+            const long id = (long)j * NZ6 + k;
+            const double zz = s->z[id], yy = s->y[j];
+            xi_weights(s, w7, zz, yy - ms, k);       set7(s->xiw[XI3], nyz, id, w7);
+            xi_weights(s, w7, zz, yy + ms, k);       set7(s->xiw[XI4], nyz, id, w7);
+            xi_weights(s, w7, zz - ms, yy, k);       set7(s->xiw[XI5], nyz, id, w7);
+            xi_weights(s, w7, zz + ms, yy, k);       set7(s->xiw[XI6], nyz, id, w7);
+            xi_weights(s, w7, zz - ms, yy - ms, k);  set7(s->xiw[XI15], nyz, id, w7);
+            xi_weights(s, w7, zz - ms, yy + ms, k);  set7(s->xiw[XI16], nyz, id, w7);
+            xi_weights(s, w7, zz + ms, yy - ms, k);  set7(s->xiw[XI17], nyz, id, w7);
+            xi_weights(s, w7, zz + ms, yy + ms, k);  set7(s->xiw[XI18], nyz, id, w7);
+        }
+    }
+
+    /* BFLInitialization: initialization.h:260-333. */
+    const long nb = 2L * NY6;
+    for (int b = 0; b < NB; ++b) { s->bflag[b] = (int*)calloc((size_t)nb, sizeof(int)); s->bq[b] = zalloc(nb); }
+    for (int b = 0; b < NBW; ++b) s->bw[b] = zalloc(7 * nb);
+    if (!p->flat) {
+        const double ly = p->ly, al = p->hill_alpha;
+        for (int k = 0; k < 2; ++k) {
+            for (int j = 3; j < NY6 - 3; ++j) {
+                const long id = (long)k * NY6 + j;
+                const double zz = s->z[j * NZ6 + k + 3], yy = s->y[j];
+                /* IsBFLBCNeeded: initializationTool.h:81-88 */
+                s->bflag[B3][id] = or_hill(yy - ms, ly, al, 0) > zz;
+                s->bflag[B4][id] = or_hill(yy + ms, ly, al, 0) > zz;
+                s->bflag[B15][id] = or_hill(yy - ms, ly, al, 0) > zz - ms;
+                s->bflag[B16][id] = or_hill(yy + ms, ly, al, 0) > zz - ms;
+            }
+        }
+        const double xm = s->x[NX6 / 2];
+        for (int k = 0; k < 2; ++k) {
+            for (int j = 3; j < NY6 - 3; ++j) {
+                const long id = (long)k * NY6 + j;
+                const double zz = s->z[j * NZ6 + k + 3], yy = s->y[j];
+                double delta;
+                if (s->bflag[B3][id]) {
+                    delta = or_delta_horizontal(zz, yy - ms, yy, yy, ly, al, ms);
+                    s->bq[B3][id] = (ms - delta) / 2.0 / ms;
+                    lagrange7(w7, xm - delta, s->x, NX6 / 2 - 3); set7(s->bw[BX38], nb, id, w7);
+                    lagrange7(w7, xm + delta, s->x, NX6 / 2 - 3); set7(s->bw[BX37], nb, id, w7);
+                    lagrange7(w7, yy + delta, s->y, j - 3);       set7(s->bw[BY378], nb, id, w7);
+                    xi_weights(s, w7, zz, yy + delta, k + 3);     set7(s->bw[BXI378], nb, id, w7);
+                }
+                if (s->bflag[B4][id]) {
+                    delta = or_delta_horizontal(zz, yy + ms, yy, yy, ly, al, ms);
+                    s->bq[B4][id] = (ms - delta) / 2.0 / ms;
+                    lagrange7(w7, xm + delta, s->x, NX6 / 2 - 3); set7(s->bw[BX49], nb, id, w7);
+                    lagrange7(w7, xm - delta, s->x, NX6 / 2 - 3); set7(s->bw[BX410], nb, id, w7);
+                    lagrange7(w7, yy - delta, s->y, j - 3);       set7(s->bw[BY4910], nb, id, w7);
+                    xi_weights(s, w7, zz, yy - delta, k + 3);     set7(s->bw[BXI4910], nb, id, w7);
+                }
+                if (s->bflag[B15][id]) {
+                    delta = or_delta_45(zz, yy, yy, yy - ms, ly, al, ms);
+                    s->bq[B15][id] = (ms - delta) / 2.0 / ms;
+                    lagrange7(w7, yy + delta, s->y, j - 3);           set7(s->bw[BY15], nb, id, w7);
+                    xi_weights(s, w7, zz + delta, yy + delta, k + 3); set7(s->bw[BXI15], nb, id, w7);
+                }
+                if (s->bflag[B16][id]) {
+                    delta = or_delta_45(zz, yy, yy, yy + ms, ly, al, ms);
+                    s->bq[B16][id] = (ms - delta) / 2.0 / ms;
+                    lagrange7(w7, yy - delta, s->y, j - 3);           set7(s->bw[BY16], nb, id, w7);
+                    xi_weights(s, w7, zz + delta, yy - delta, k + 3); set7(s->bw[BXI16], nb, id, w7);
+                }
+            }
+        }
+    }
+
+    for (int b = 0; b < 2; ++b)
+        for (int q = 0; q < NQ; ++q) s->f[b][q] = zalloc(s->ntot);
+    s->rho = zalloc(s->ntot); s->u = zalloc(s->ntot); s->v = zalloc(s->ntot); s->w = zalloc(s->ntot);
+    s->ub = zalloc((long)NZ6 * NX6);
+    /* InitialUsingDftFunc: initialization.h:46 */
+    s->force = (8.0 * s->nu * s->uref) / (p->lz * p->lz) * 5.0;
+    s->rho_modify = 0.0;
+    return s;
+}
+
+void or_destroy(or_state* s)
+{
+    if (!s) return;
+    free(s->x); free(s->y); free(s->z); free(s->xi);
+    free(s->xp0); free(s->xp2); free(s->yp0); free(s->yp2);
+    for (int f = 0; f < NXI; ++f) free(s->xiw[f]);
+    for (int b = 0; b < NB; ++b) { free(s->bflag[b]); free(s->bq[b]); }
+    for (int b = 0; b < NBW; ++b) free(s->bw[b]);
+    for (int b = 0; b < 2; ++b)
+        for (int q = 0; q < NQ; ++q) free(s->f[b][q]);
+    free(s->rho); free(s->u); free(s->v); free(s->w); free(s->ub);
+    free(s);
+}
+
+int or_threads(int n)
+{
+#ifdef _OPENMP
+    int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+void or_scalars(const or_state* s, double* o)
+{
+    o[0] = s->dt; o[1] = s->tau; o[2] = s->nu; o[3] = s->uref;
+    o[4] = s->a; o[5] = s->dx; o[6] = s->dy; o[7] = s->force;
+}
+
+/* padded index of a unique node, with periodic wrap in x and y */
+static inline long pidx(const or_state* s, int i, int j, int k)
+{
+    return (long)(j + 3) * s->nface + (long)(k + 3) * s->NX6 + (i + 3);
+}
+
+static inline int wrap(int a, int n) { a %= n; return a < 0 ? a + n : a; }
+
+/* Scatter a unique-node field into every padded node (interior, duplicate and ghost)
+ * that is a periodic image of it; z ghosts are never read and stay untouched. */
+static void scatter(const or_state* s, double* dst, const double* src_unique)
+{
+    const or_params* p = &s->p;
+    for (int J = 0; J < s->NY6; ++J)
+        for (int K = 3; K < s->NZ6 - 3; ++K)
+            for (int I = 0; I < s->NX6; ++I) {
+                int i = wrap(I - 3, p->nx), j = wrap(J - 3, p->ny), k = K - 3;
+                dst[(long)J * s->nface + (long)K * s->NX6 + I] =
+                    src_unique[((long)k * p->ny + j) * p->nx + i];
+            }
+}
+
+static void gather(const or_state* s, double* dst_unique, const double* src)
+{
+    const or_params* p = &s->p;
+    for (int k = 0; k < p->nz; ++k)
+        for (int j = 0; j < p->ny; ++j)
+            for (int i = 0; i < p->nx; ++i)
+                dst_unique[((long)k * p->ny + j) * p->nx + i] = src[pidx(s, i, j, k)];
+}
+
+/* SplitMix64 finaliser on a counter: perturbation of node (i,j,k), component c. */
+static double unit_rand(uint64_t seed, uint64_t index)
+{
+    uint64_t z = seed + (index + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* InitialUsingDftFunc: initialization.h:6-44, with the seeded velocity perturbation. */
+void or_init_equilibrium(or_state* s, double rho0, const double u0[3], double amp, uint64_t seed)
+{
+    const or_params* p = &s->p;
+    const long n = (long)p->nx * p->ny * p->nz;
+    double* fu = zalloc(NQ * n);
+    double* mac = zalloc(4 * n);
+    for (int j = 0; j < p->ny; ++j)
+        for (int k = 0; k < p->nz; ++k)
+            for (int i = 0; i < p->nx; ++i) {
+                const long id = ((long)k * p->ny + j) * p->nx + i;
+                const uint64_t base = (((uint64_t)j * p->nz + k) * p->nx + i) * 3;
+                double uu[3];
+                for (int c = 0; c < 3; ++c)
+                    uu[c] = u0[c] + (amp != 0.0 ? amp * (2.0 * unit_rand(seed, base + c) - 1.0) : 0.0);
+                const double r = rho0;
+                const double udot = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
+                fu[id] = W[0] * r * (1.0 - 1.5 * udot);
+                for (int d = 1; d < NQ; ++d) {
+                    const double eu = E[d][0] * uu[0] + E[d][1] * uu[1] + E[d][2] * uu[2];
+                    fu[(long)d * n + id] = W[d] * r * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * udot);
+                }
+                mac[id] = r; mac[n + id] = uu[0]; mac[2 * n + id] = uu[1]; mac[3 * n + id] = uu[2];
+            }
+    for (int b = 0; b < 2; ++b)
+        for (int q = 0; q < NQ; ++q) scatter(s, s->f[b][q], fu + (long)q * n);
+    scatter(s, s->rho, mac); scatter(s, s->u, mac + n); scatter(s, s->v, mac + 2 * n); scatter(s, s->w, mac + 3 * n);
+    free(fu); free(mac);
+    s->cur = 0;
+}
+
+void or_set_f(or_state* s, const double* fu)
+{
+    const long n = (long)s->p.nx * s->p.ny * s->p.nz;
+    for (int q = 0; q < NQ; ++q) scatter(s, s->f[s->cur][q], fu + (long)q * n);
+}
+
+void or_get_f(const or_state* s, double* fu)
+{
+    const long n = (long)s->p.nx * s->p.ny * s->p.nz;
+    for (int q = 0; q < NQ; ++q) gather(s, fu + (long)q * n, s->f[s->cur][q]);
+}
+
+void or_get_macros(const or_state* s, double* rho, double* u, double* v, double* w)
+{
+    if (rho) gather(s, rho, s->rho);
+    if (u) gather(s, u, s->u);
+    if (v) gather(s, v, s->v);
+    if (w) gather(s, w, s->w);
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Interpolation (interpolationHillISLBM.h:4-161).  Intrpl7 sums f1*a1 + ... + f7*a7 left */
+/* to right; each helper keeps the reference's inner-to-outer axis order.               */
+/* ---------------------------------------------------------------------------------- */
+
+static inline double sum7(const double* f, long base, long stride, const double* w, long ws, long wi)
+{
+    double r = f[base] * w[wi];
+    for (int m = 1; m < 7; ++m) r = r + f[base + m * stride] * w[m * ws + wi];
+    return r;
+}
+
+/* F1/F2_Intrpl7 (x only): :14-20 */
+static inline double ip_x(const double* f, long base, const double* xw, long xs, long ix)
+{
+    return sum7(f, base, 1, xw, xs, ix);
+}
+
+/* F5/F6_Intrpl7 (xi only): :60-66 */
+static inline double ip_xi(const double* f, long base, long nline, const double* zw, long zs, long iz)
+{
+    return sum7(f, base, nline, zw, zs, iz);
+}
+
+/* F3/F4_Intrpl7 (xi inner, y outer): :22-58 */
+static inline double ip_xi_y(const double* f, long base, long nline, long nface,
+                             const double* yw, long ys, long iy, const double* zw, long zs, long iz)
+{
+    double r = 0.0;
+    for (int n = 0; n < 7; ++n) {
+        double in = sum7(f, base + n * nface, nline, zw, zs, iz);
+        r = (n == 0) ? in * yw[iy] : r + in * yw[n * ys + iy];
+    }
+    return r;
+}
+
+/* Y_XI_Intrpl7 (y inner, xi outer): :150-161 */
+static inline double ip_y_xi(const double* f, long base, long nline, long nface,
+                             const double* yw, long ys, long iy, const double* zw, long zs, long iz)
+{
+    double r = 0.0;
+    for (int m = 0; m < 7; ++m) {
+        double in = sum7(f, base + m * nline, nface, yw, ys, iy);
+        r = (m == 0) ? in * zw[iz] : r + in * zw[m * zs + iz];
+    }
+    return r;
+}
+
+/* X_XI_Intrpl7 (x inner, xi outer): :137-148 */
+static inline double ip_x_xi(const double* f, long base, long nline,
+                             const double* xw, long xs, long ix, const double* zw, long zs, long iz)
+{
+    double r = 0.0;
+    for (int m = 0; m < 7; ++m) {
+        double in = sum7(f, base + m * nline, 1, xw, xs, ix);
+        r = (m == 0) ? in * zw[iz] : r + in * zw[m * zs + iz];
+    }
+    return r;
+}
+
+/* X_Y_XI_Intrpl7 (x inner, y middle, xi outer): :68-135 */
+static inline double ip_x_y_xi(const double* f, long base, long nline, long nface,
+                               const double* xw, long xs, long ix, const double* yw, long ys, long iy,
+                               const double* zw, long zs, long iz)
+{
+    double r = 0.0;
+    for (int m = 0; m < 7; ++m) {
+        double mid = 0.0;
+        for (int n = 0; n < 7; ++n) {
+            double in = sum7(f, base + m * nline + n * nface, 1, xw, xs, ix);
+            mid = (n == 0) ? in * yw[iy] : mid + in * yw[n * ys + iy];
+        }
+        r = (m == 0) ? mid * zw[iz] : r + mid * zw[m * zs + iz];
+    }
+    return r;
+}
+
+/* Smagorinsky extension (not in the reference code; LES/2.Hou et al. (1996).tex:489-518):
+ * returns tau* = tau0 + 3 nu_t/dt with nu_t = (Cs Delta)^2 |S|, |S| = sqrt(2 S:S) solved in
+ * closed form from Q = Pi_neq:Pi_neq, Pi_neq = sum_i e_i e_i (f_i - f_i^eq). */
+static double les_tau(const double F[NQ], const double Feq[NQ], double rho, double tau0,
+                      double dt, double cs, double delta)
+{
+    double P[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 1; q < NQ; ++q) {
+        const double d = F[q] - Feq[q];
+        P[0] += E[q][0] * E[q][0] * d; P[1] += E[q][1] * E[q][1] * d; P[2] += E[q][2] * E[q][2] * d;
+        P[3] += E[q][0] * E[q][1] * d; P[4] += E[q][1] * E[q][2] * d; P[5] += E[q][0] * E[q][2] * d;
+    }
+    const double Q = P[0] * P[0] + P[1] * P[1] + P[2] * P[2] + 2.0 * (P[3] * P[3] + P[4] * P[4] + P[5] * P[5]);
+    const double C = cs * cs * delta * delta;
+    const double b = tau0 * dt;
+    const double S = (sqrt(b * b + 18.0 * sqrt(2.0) * C * sqrt(Q) / rho) - b) / (6.0 * C);
+    return tau0 + 3.0 * C * S / dt;
+}
+
+/* One node of stream_collide: evolution.h:398-683. */
+static void node_update(or_state* s, double* const* fo, double* const* fn, int I, int J, int K)
+{
+    const int NX6 = s->NX6, NY6 = s->NY6, NZ6 = s->NZ6;
+    const long nface = s->nface, nline = NX6;
+    const long index = (long)J * nface + (long)K * nline + I;
+    const long nyz = (long)NY6 * NZ6;
+    long ixi = (long)J * NZ6 + K;
+    int cz = K - 3;
+    if (K <= 6) cz = 3;
+    if (K >= NZ6 - 7) cz = NZ6 - 10;
+    double F[NQ];
+
+    /* interpolated streaming: evolution.h:451-475 */
+    F[0] = fo[0][index];
+    F[1] = ip_x(fo[1], (long)J * nface + K * nline + (I - 3), s->xp0, NX6, I);
+    F[2] = ip_x(fo[2], (long)J * nface + K * nline + (I - 3), s->xp2, NX6, I);
+    F[3] = ip_xi_y(fo[3], (long)(J - 3) * nface + cz * nline + I, nline, nface, s->yp0, NY6, J, s->xiw[XI3], nyz, ixi);
+    F[4] = ip_xi_y(fo[4], (long)(J - 3) * nface + cz * nline + I, nline, nface, s->yp2, NY6, J, s->xiw[XI4], nyz, ixi);
+    F[5] = ip_xi(fo[5], (long)J * nface + cz * nline + I, nline, s->xiw[XI5], nyz, ixi);
+    F[6] = ip_xi(fo[6], (long)J * nface + cz * nline + I, nline, s->xiw[XI6], nyz, ixi);
+    const long b3 = (long)(J - 3) * nface + cz * nline + (I - 3);
+    F[7] = ip_x_y_xi(fo[7], b3, nline, nface, s->xp0, NX6, I, s->yp0, NY6, J, s->xiw[XI3], nyz, ixi);
+    F[8] = ip_x_y_xi(fo[8], b3, nline, nface, s->xp2, NX6, I, s->yp0, NY6, J, s->xiw[XI3], nyz, ixi);
+    F[9] = ip_x_y_xi(fo[9], b3, nline, nface, s->xp0, NX6, I, s->yp2, NY6, J, s->xiw[XI4], nyz, ixi);
+    F[10] = ip_x_y_xi(fo[10], b3, nline, nface, s->xp2, NX6, I, s->yp2, NY6, J, s->xiw[XI4], nyz, ixi);
+    const long bxz = (long)J * nface + cz * nline + (I - 3);
+    F[11] = ip_x_xi(fo[11], bxz, nline, s->xp0, NX6, I, s->xiw[XI5], nyz, ixi);
+    F[12] = ip_x_xi(fo[12], bxz, nline, s->xp2, NX6, I, s->xiw[XI5], nyz, ixi);
+    F[13] = ip_x_xi(fo[13], bxz, nline, s->xp0, NX6, I, s->xiw[XI6], nyz, ixi);
+    F[14] = ip_x_xi(fo[14], bxz, nline, s->xp2, NX6, I, s->xiw[XI6], nyz, ixi);
+    const long byz = (long)(J - 3) * nface + cz * nline + I;
+    F[15] = ip_y_xi(fo[15], byz, nline, nface, s->yp0, NY6, J, s->xiw[XI15], nyz, ixi);
+    F[16] = ip_y_xi(fo[16], byz, nline, nface, s->yp2, NY6, J, s->xiw[XI16], nyz, ixi);
+    F[17] = ip_y_xi(fo[17], byz, nline, nface, s->yp0, NY6, J, s->xiw[XI17], nyz, ixi);
+    F[18] = ip_y_xi(fo[18], byz, nline, nface, s->yp2, NY6, J, s->xiw[XI18], nyz, ixi);
+
+    /* flat-wall half-way bounce-back with pre-collision f_old (Q6): evolution.h:482-496 */
+    if (K == 3) {
+        F[5] = fo[6][index]; F[11] = fo[14][index]; F[12] = fo[13][index];
+        F[15] = fo[18][index]; F[16] = fo[17][index];
+    }
+    if (K == NZ6 - 4) {
+        F[6] = fo[5][index]; F[13] = fo[12][index]; F[14] = fo[11][index];
+        F[17] = fo[16][index]; F[18] = fo[15][index];
+    }
+
+    /* curved-wall BFL at K = 3, 4: evolution.h:505-572 (weights at (K-3)*NY6+J, Q3) */
+    if (K == 3 || K == 4) {
+        const long ib = (long)(K - 3) * NY6 + J;
+        const long nb = 2L * NY6;
+        const long bY = (long)(J - 3) * nface + 3 * nline + I;
+        const long bXY = (long)(J - 3) * nface + 3 * nline + (I - 3);
+        if (s->bflag[B3][ib] == 1) {
+            const double q = s->bq[B3][ib];
+            if (q > 0.5) {
+                F[3] = (1 / (2 * q)) * fo[4][index] + ((2 * q - 1) / (2 * q)) * fo[3][index];
+                F[7] = (1 / (2 * q)) * fo[10][index] + ((2 * q - 1) / (2 * q)) * fo[7][index];
+                F[8] = (1 / (2 * q)) * fo[9][index] + ((2 * q - 1) / (2 * q)) * fo[8][index];
+            }
+            if (q < 0.5) {
+                F[3] = ip_y_xi(fo[4], bY, nline, nface, s->bw[BY378], nb, ib, s->bw[BXI378], nb, ib);
+                F[7] = ip_x_y_xi(fo[10], bXY, nline, nface, s->bw[BX37], nb, ib, s->bw[BY378], nb, ib, s->bw[BXI378], nb, ib);
+                F[8] = ip_x_y_xi(fo[9], bXY, nline, nface, s->bw[BX38], nb, ib, s->bw[BY378], nb, ib, s->bw[BXI378], nb, ib);
+            }
+        }
+        if (s->bflag[B4][ib] == 1) {
+            const double q = s->bq[B4][ib];
+            if (q > 0.5) {
+                F[4] = (1 / (2 * q)) * fo[3][index] + ((2 * q - 1) / (2 * q)) * fo[4][index];
+                F[9] = (1 / (2 * q)) * fo[8][index] + ((2 * q - 1) / (2 * q)) * fo[9][index];
+                F[10] = (1 / (2 * q)) * fo[7][index] + ((2 * q - 1) / (2 * q)) * fo[10][index];
+            }
+            if (q < 0.5) {
+                F[4] = ip_y_xi(fo[3], bY, nline, nface, s->bw[BY4910], nb, ib, s->bw[BXI4910], nb, ib);
+                F[9] = ip_x_y_xi(fo[8], bXY, nline, nface, s->bw[BX49], nb, ib, s->bw[BY4910], nb, ib, s->bw[BXI4910], nb, ib);
+                F[10] = ip_x_y_xi(fo[7], bXY, nline, nface, s->bw[BX410], nb, ib, s->bw[BY4910], nb, ib, s->bw[BXI4910], nb, ib);
+            }
+        }
+        if (s->bflag[B15][ib] == 1) {
+            const double q = s->bq[B15][ib];
+            if (q > 0.5) F[15] = (1 / (2 * q)) * fo[18][index] + ((2 * q - 1) / (2 * q)) * fo[15][index];
+            if (q < 0.5) F[15] = ip_y_xi(fo[18], bY, nline, nface, s->bw[BY15], nb, ib, s->bw[BXI15], nb, ib);
+        }
+        if (s->bflag[B16][ib] == 1) {
+            const double q = s->bq[B16][ib];
+            if (q > 0.5) F[16] = (1 / (2 * q)) * fo[17][index] + ((2 * q - 1) / (2 * q)) * fo[16][index];
+            if (q < 0.5) F[16] = ip_y_xi(fo[17], bY, nline, nface, s->bw[BY16], nb, ib, s->bw[BXI16], nb, ib);
+        }
+    }
+
+    /* mass fix and macroscopic quantities: evolution.h:597-606 */
+    F[0] = F[0] + s->rho_modify;
+    double rho_s = F[0];
+    for (int q = 1; q < NQ; ++q) rho_s = rho_s + F[q];
+    const double u1 = (F[1] + F[7] + F[9] + F[11] + F[13] - (F[2] + F[8] + F[10] + F[12] + F[14])) / rho_s;
+    const double v1 = (F[3] + F[7] + F[8] + F[15] + F[17] - (F[4] + F[9] + F[10] + F[16] + F[18])) / rho_s;
+    const double w1 = (F[5] + F[11] + F[12] + F[15] + F[16] - (F[6] + F[13] + F[14] + F[17] + F[18])) / rho_s;
+    const double udot = u1 * u1 + v1 * v1 + w1 * w1;
+
+    /* equilibrium: evolution.h:608-626 */
+    double Feq[NQ];
+    Feq[0] = (1. / 3.) * rho_s * (1.0 - 1.5 * udot);
+    for (int q = 1; q < NQ; ++q) {
+        const double eu = (E[q][0] ? E[q][0] * u1 : 0.0) + (E[q][1] ? E[q][1] * v1 : 0.0) + (E[q][2] ? E[q][2] * w1 : 0.0);
+        Feq[q] = W[q] * rho_s * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * udot);
+    }
+
+    /* MRT: MRT_Matrix.h:4-23 (rates), MRT_Process.h:4-179 (m, meq, collision + force) */
+    double S[NQ] = {0.0, 1.19, 1.4, 0.0, 1.2, 0.0, 1.2, 0.0, 1.2, 0.0,
+                    1.4, 0.0, 1.4, 0.0, 0.0, 0.0, 1.5, 1.5, 1.5};
+    double tau_eff = s->tau;
+    if (s->p.cs_smag > 0.0) {
+        const double dzl = (s->z[(long)J * NZ6 + K + 1] - s->z[(long)J * NZ6 + K - 1]) / 2.0;
+        const double delta = cbrt(s->dx * s->dy * dzl);
+        tau_eff = les_tau(F, Feq, rho_s, s->tau, s->dt, s->p.cs_smag, delta);
+    }
+    S[9] = S[11] = S[13] = S[14] = S[15] = 1.0 / tau_eff;
+    double m[NQ], meq[NQ];
+    for (int a = 0; a < NQ; ++a) {
+        double r = F[0] * s->M[a][0], re = Feq[0] * s->M[a][0];
+        for (int b = 1; b < NQ; ++b) { r = r + F[b] * s->M[a][b]; re = re + Feq[b] * s->M[a][b]; }
+        m[a] = r; meq[a] = re;
+    }
+    const double fy = s->force;
+    for (int q = 0; q < NQ; ++q) {
+        double r = F[q];
+        for (int a = 0; a < NQ; ++a) r = r - s->Mi[q][a] * S[a] * (m[a] - meq[a]);
+        if (E[q][1] != 0) {
+            const double c = (E[q][0] == 0 && E[q][2] == 0) ? 1.0 / 6 : 1.0 / 12;
+            r = (E[q][1] > 0) ? r + c * fy * s->dt : r - c * fy * s->dt;
+        }
+        fn[q][index] = r;
+    }
+    s->rho[index] = rho_s;
+    s->u[index] = u1;
+    s->v[index] = v1;
+    s->w[index] = w1;
+}
+
+/* periodicSW (evolution.h:732-777) and the single-rank y halo (communication.h:11-82 with
+ * l_nbr = r_nbr = self): ghosts are periodic images; y first, then x over all planes. */
+static void fill_ghosts(or_state* s, double* const* f)
+{
+    const int NX = s->p.nx, NY = s->p.ny, NX6 = s->NX6, NY6 = s->NY6, NZ6 = s->NZ6;
+    const long nface = s->nface;
+    double* arr[NQ + 4];
+    for (int q = 0; q < NQ; ++q) arr[q] = f[q];
+    arr[NQ] = s->rho; arr[NQ + 1] = s->u; arr[NQ + 2] = s->v; arr[NQ + 3] = s->w;
+    for (int a = 0; a < NQ + 4; ++a) {
+        double* g = arr[a];
+        for (int J = 0; J < 3; ++J) {
+            memcpy(g + (long)J * nface, g + (long)(J + NY) * nface, sizeof(double) * nface);
+            memcpy(g + (long)(NY6 - 1 - J) * nface, g + (long)(NY6 - 1 - J - NY) * nface, sizeof(double) * nface);
+        }
+        for (int J = 0; J < NY6; ++J)
+            for (int K = 0; K < NZ6; ++K) {
+                double* row = g + (long)J * nface + (long)K * NX6;
+                for (int i = 0; i < 3; ++i) {
+                    row[i] = row[i + NX];
+                    row[NX6 - 1 - i] = row[NX6 - 1 - i - NX];
+                }
+            }
+    }
+}
+
+/* AccumulateUbulk: evolution.h:826-840 on the ghost plane J = 0 (Q7). */
+static void accumulate_ubulk(or_state* s)
+{
+    const int NX6 = s->NX6, NZ6 = s->NZ6;
+    for (int K = 3; K < NZ6 - 3; ++K)
+        for (int I = 3; I < NX6 - 3; ++I) {
+            const double dxl = (s->x[I + 1] - s->x[I - 1]) / 2.0;
+            const double dzl = (s->z[0 * NZ6 + K + 1] - s->z[0 * NZ6 + K - 1]) / 2.0;
+            s->ub[K * NX6 + I] += s->v[0 * s->nface + (long)K * NX6 + I] * dxl * dzl;
+        }
+}
+
+void or_modify_forcing(or_state* s)
+{
+    /* Launch_ModifyForcingTerm: evolution.h:1003-1042 (jp = 1: the rank average is F). */
+    const int NX6 = s->NX6, NZ6 = s->NZ6;
+    double ub = 0.0;
+    for (int K = 3; K < NZ6 - 3; ++K)
+        for (int I = 3; I < NX6 - 4; ++I) {
+            ub = ub + s->ub[K * NX6 + I];
+            s->ub[K * NX6 + I] = 0.0;
+        }
+    ub = ub / (double)(s->p.lx * (s->p.lz - 1.0)) / s->p.ndtfrc;
+    const double beta = (0.001 > 3.0 / s->p.re) ? 0.001 : 3.0 / s->p.re;
+    s->force = s->force + beta * (s->uref - ub) * s->uref / 3.036;
+}
+
+void or_step(or_state* s, int nsteps)
+{
+    for (int n = 0; n < nsteps; ++n) {
+        accumulate_ubulk(s);
+        double* const* fo = s->f[s->cur];
+        double* const* fn = s->f[1 - s->cur];
+        const int NX6 = s->NX6, NY6 = s->NY6, NZ6 = s->NZ6;
+#pragma omp parallel for schedule(static)
+        for (int J = 3; J < NY6 - 3; ++J)
+            for (int K = 3; K < NZ6 - 3; ++K)
+                for (int I = 3; I < NX6 - 3; ++I) node_update(s, fo, fn, I, J, K);
+        fill_ghosts(s, fn);
+        s->cur = 1 - s->cur;
+        s->step++;
+        if (s->p.ndtfrc > 0 && s->step % s->p.ndtfrc == 0) or_modify_forcing(s);
+    }
+}
+
+double or_get_force(const or_state* s) { return s->force; }
+void or_set_force(or_state* s, double f) { s->force = f; }
+void or_set_rho_modify(or_state* s, double dr) { s->rho_modify = dr; }
+long or_step_count(const or_state* s) { return s->step; }
+
+double or_flux_sum(const or_state* s)
+{
+    double ub = 0.0;
+    for (int K = 3; K < s->NZ6 - 3; ++K)
+        for (int I = 3; I < s->NX6 - 4; ++I) ub += s->ub[K * s->NX6 + I];
+    return ub;
+}
+
+/* Unique-node geometry tables for tests:
+ *   x[nx] y[ny] xi[nz] z[ny][nz] (j-major)
+ *   xw0/xw2 [nx][7], yw0/yw2 [ny][7]
+ *   xiw3..xiw18 [nz][ny][7]
+ *   bfl_flag3/4/15/16, bfl_q3/4/15/16 [2][ny]
+ *   bfl_<name> [2][ny][7]  (x37 x38 y378 xi378 x49 x410 y4910 xi4910 y15 xi15 y16 xi16) */
+int or_get_table(const or_state* s, const char* name, double* out, int cap)
+{
+    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+    const int NX6 = s->NX6, NY6 = s->NY6, NZ6 = s->NZ6;
+    const long nyz = (long)NY6 * NZ6;
+    int n = 0;
+#define NEED(c) do { if ((c) > cap) return -(c); } while (0)
+    if (!strcmp(name, "x")) { NEED(nx); for (int i = 0; i < nx; ++i) out[n++] = s->x[i + 3]; return n; }
+    if (!strcmp(name, "y")) { NEED(ny); for (int j = 0; j < ny; ++j) out[n++] = s->y[j + 3]; return n; }
+    if (!strcmp(name, "xi")) { NEED(nz); for (int k = 0; k < nz; ++k) out[n++] = s->xi[k + 3]; return n; }
+    if (!strcmp(name, "z")) {
+        NEED(ny * nz);
+        for (int j = 0; j < ny; ++j) for (int k = 0; k < nz; ++k) out[n++] = s->z[(long)(j + 3) * NZ6 + k + 3];
+        return n;
+    }
+    const double* xt = !strcmp(name, "xw0") ? s->xp0 : !strcmp(name, "xw2") ? s->xp2 : NULL;
+    if (xt) { NEED(nx * 7); for (int i = 0; i < nx; ++i) for (int m = 0; m < 7; ++m) out[n++] = xt[(long)m * NX6 + i + 3]; return n; }
+    const double* yt = !strcmp(name, "yw0") ? s->yp0 : !strcmp(name, "yw2") ? s->yp2 : NULL;
+    if (yt) { NEED(ny * 7); for (int j = 0; j < ny; ++j) for (int m = 0; m < 7; ++m) out[n++] = yt[(long)m * NY6 + j + 3]; return n; }
+    static const char* xin[NXI] = {"xiw3", "xiw4", "xiw5", "xiw6", "xiw15", "xiw16", "xiw17", "xiw18"};
+    for (int f = 0; f < NXI; ++f)
+        if (!strcmp(name, xin[f])) {
+            NEED(nz * ny * 7);
+            for (int k = 0; k < nz; ++k) for (int j = 0; j < ny; ++j) for (int m = 0; m < 7; ++m)
+                out[n++] = s->xiw[f][(long)m * nyz + (long)(j + 3) * NZ6 + k + 3];
+            return n;
+        }
+    static const char* fn[NB] = {"bfl_flag3", "bfl_flag4", "bfl_flag15", "bfl_flag16"};
+    static const char* qn[NB] = {"bfl_q3", "bfl_q4", "bfl_q15", "bfl_q16"};
+    for (int b = 0; b < NB; ++b) {
+        if (!strcmp(name, fn[b])) { NEED(2 * ny); for (int k = 0; k < 2; ++k) for (int j = 0; j < ny; ++j) out[n++] = s->bflag[b][(long)k * NY6 + j + 3]; return n; }
+        if (!strcmp(name, qn[b])) { NEED(2 * ny); for (int k = 0; k < 2; ++k) for (int j = 0; j < ny; ++j) out[n++] = s->bq[b][(long)k * NY6 + j + 3]; return n; }
+    }
+    static const char* wn[NBW] = {"bfl_x37", "bfl_x38", "bfl_y378", "bfl_xi378", "bfl_x49", "bfl_x410",
+                                  "bfl_y4910", "bfl_xi4910", "bfl_y15", "bfl_xi15", "bfl_y16", "bfl_xi16"};
+    for (int b = 0; b < NBW; ++b)
+        if (!strcmp(name, wn[b])) {
+            NEED(2 * ny * 7);
+            for (int k = 0; k < 2; ++k) for (int j = 0; j < ny; ++j) for (int m = 0; m < 7; ++m)
+                out[n++] = s->bw[b][(long)m * 2 * NY6 + (long)k * NY6 + j + 3];
+            return n;
+        }
+#undef NEED
+    return 0;
+}

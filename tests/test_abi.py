@@ -1,0 +1,71 @@
+"""The C-ABI library loads, exports every function include/islbm.h declares, and rejects bad
+input before touching a GPU.  No kernel runs here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "islbm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:int|void|const char\*)\s+\**\s*([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("islbm_create", "islbm_step", "Launch_CollisionStreaming", "Launch_ModifyForcingTerm"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    L = pkg.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_default_params_are_the_reference_set(pkg):
+    p = pkg.default_params()
+    assert (p.nx, p.ny, p.nz) == (32, 128, 64)
+    assert (p.lx, p.ly, p.lz, p.cfl, p.lxi) == (4.5, 9.0, 3.036, 0.6, 10.0)
+    assert (p.tau, p.re, p.ndtfrc) == (0.6833, 50.0, 10000)
+    assert p.mesh_mode == pkg.MESH_BISECT and p.precision == pkg.FP64
+
+
+def test_required_bytes(pkg):
+    p = pkg.default_params(nx=128, ny=256, nz=128, precision="fp32")
+    n = 128 * 256 * 128
+    b = pkg.required_bytes(p)
+    assert 2 * 19 * n * 4 + 4 * n * 4 <= b < 2 * 19 * n * 4 + 4 * n * 4 + 32 * 2 ** 20
+    p16 = pkg.default_params(nx=128, ny=256, nz=128, precision="bf16")
+    assert pkg.required_bytes(p16) < b
+
+
+@pytest.mark.parametrize("bad", [dict(nx=2), dict(nz=4), dict(re=0.0), dict(tau=0.0, u_ref=0.0),
+                                 dict(tau=0.5), dict(ndtfrc=0), dict(precision=7), dict(cs_smag=-1.0),
+                                 dict(mesh_mode=5), dict(lz=0.9)])
+def test_create_rejects_bad_parameters(pkg, bad):
+    p = pkg.default_params(**bad)
+    with pytest.raises(pkg.ISLBMError) as e:
+        pkg.Solver(p)
+    assert "(-1)" in str(e.value)
+
+
+def test_create_refuses_over_budget(pkg):
+    p = pkg.default_params(nx=128, ny=256, nz=128, precision="fp32")
+    with pytest.raises(pkg.ISLBMError) as e:
+        pkg.Solver(p, hbm_budget=1 << 20)
+    assert "(-3)" in str(e.value) and "budget" in str(e.value)
+
+
+def test_reference_entry_point_without_init_records_error(pkg):
+    L = pkg.lib()
+    arr = (ctypes.c_void_p * 19)()
+    L.islbm_ref_shutdown()
+    L.Launch_CollisionStreaming(arr, arr)
+    assert b"islbm_ref_init" in L.islbm_last_error()

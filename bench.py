@@ -30,10 +30,13 @@ HBM_PEAK_GBS = 8000.0
 
 GRID = dict(nx=128, ny=256, nz=128)
 PHYSICS = {
-    # Re_h 2800, U_b 0.05, Smagorinsky Cs 0.1 on the bisected (first cell = minSize) mesh
-    "production": dict(tau=0.0, re=2800.0, u_ref=0.05, cs_smag=0.1, mesh_mode=0),
-    # the reference's shipped operating point (tau 0.6833, Re 50)
+    # the reference's shipped operating point (tau 0.6833, Re 50) with the Smagorinsky
+    # relaxation evaluated (Cs 0.1): the LES cost is in the step, the state stays bounded
+    "reference-les": dict(tau=0.6833, re=50.0, u_ref=0.0, cs_smag=0.1, mesh_mode=0),
+    # the reference's shipped operating point, plain MRT
     "reference": dict(tau=0.6833, re=50.0, u_ref=0.0, cs_smag=0.0, mesh_mode=0),
+    # Re_h 2800, U_b 0.05, Cs 0.1: the reference scheme diverges here within ~80 steps
+    "production": dict(tau=0.0, re=2800.0, u_ref=0.05, cs_smag=0.1, mesh_mode=0),
 }
 
 
@@ -71,11 +74,11 @@ def cpu_baseline(physics: dict, steps: int):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
     ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled"])
-    ap.add_argument("--physics", default="production", choices=sorted(PHYSICS))
+    ap.add_argument("--physics", default="reference-les", choices=sorted(PHYSICS))
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -122,6 +125,10 @@ def main() -> int:
                      "avg_launch_us": round(avg_collide_s * 1e6, 2)},
     }
     s.close()
+    try:
+        out["roofline"]["copy_gbs_measured"] = round(pkg.copy_bandwidth(1 << 30, 20), 1)
+    except Exception as e:  # diagnostic only
+        print(f"[bench] copy probe failed: {e}", file=sys.stderr)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(phys, args.cpu_steps)
     print(json.dumps(out))

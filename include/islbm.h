@@ -133,6 +133,15 @@ int islbm_time_steps(islbm_solver* s, int32_t nsteps, double out[2]);
 
 const char* islbm_last_error(void);
 
+/* Bandwidth of a plain float4 copy kernel over `bytes` (read + write counted), averaged
+ * over `iters` launches: the measured HBM reference beside the vendor peak. */
+int islbm_copy_bandwidth(int32_t device, size_t bytes, int32_t iters, double* gbps);
+
+/* Host-only consistency check of the collision operator: the opposite-pair evaluation
+ * the kernels use against the row-by-row MRT form (MRT_Process.h:4-179) on seeded states.
+ * out = max |difference| for [FP64, FP64+LES, FP32, FP32+LES].  Touches no GPU. */
+int islbm_selftest_collision(double out[4]);
+
 /* ---------------------------------------------------------------------------------- */
 /* Reference-named entry points (evolution.h:842, :1003).  They drive a process-wide   */
 /* default solver created by islbm_ref_init and take the reference's padded FP64        */

@@ -696,6 +696,40 @@ int islbm_time_steps(islbm_solver* s, int32_t nsteps, double out[2])
 
 const char* islbm_last_error(void) { return g_err.c_str(); }
 
+int islbm_copy_bandwidth(int32_t device, size_t bytes, int32_t iters, double* gbps)
+{
+    if (!gbps || iters <= 0 || bytes < 16) return fail(ISLBM_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(device));
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    };
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 0, bytes, st) != hipSuccess) {
+        cleanup();
+        return fail(ISLBM_ERR_HIP, "copy probe set-up failed");
+    }
+    hipError_t e = launch_stream_copy(st, a, b, (long long)bytes);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int t = 0; t < iters && e == hipSuccess; ++t) e = launch_stream_copy(st, a, b, (long long)bytes);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    cleanup();
+    if (e != hipSuccess) return fail(ISLBM_ERR_HIP, "copy probe: %s", hipGetErrorString(e));
+    *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+    return ISLBM_OK;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------------------------
@@ -822,3 +856,50 @@ void Launch_ModifyForcingTerm(void)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Host self-check of the collision: the opposite-pair form used by the kernels against
+// the row-by-row MRT form, on seeded near-equilibrium states (FP64 and FP32, with and
+// without the Smagorinsky relaxation).  No GPU involved.
+// ------------------------------------------------------------------------------------
+#include "islbm_step.h"
+
+namespace {
+template <bool kDev, class C>
+double collision_forms_gap(int les, uint64_t& state)
+{
+    auto uni = [&]() {  // SplitMix64 -> [-1, 1)
+        uint64_t z = (state += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    };
+    double worst = 0.0;
+    for (int it = 0; it < 500; ++it) {
+        C A[NQ], B[NQ];
+        const double ux = 0.05 * uni(), uy = 0.05 * uni(), uz = 0.05 * uni(), rho = 1.0 + 0.01 * uni();
+        for (int q = 0; q < NQ; ++q) {
+            const double eu = cx(q) * ux + cy(q) * uy + cz(q) * uz;
+            const double f = wq(q) * rho * (1 + 3 * eu + 4.5 * eu * eu - 1.5 * (ux * ux + uy * uy + uz * uz)) * (1 + 0.01 * uni());
+            A[q] = B[q] = (C)(kDev ? f - wq(q) : f);
+        }
+        C r1, u1, v1, w1, t1, r2, u2, v2, w2, t2;
+        collide_node<kDev>(A, (C)1e-5, (C)0.0190875, (C)2e-4, (C)0.05, les, (C)0.6, (C)0.1, r1, u1, v1, w1, t1);
+        collide_pairs<kDev>(B, (C)1e-5, (C)0.0190875, (C)2e-4, (C)0.05, les, (C)0.6, (C)0.1, r2, u2, v2, w2, t2);
+        for (int q = 0; q < NQ; ++q) worst = std::max(worst, std::fabs((double)A[q] - (double)B[q]));
+    }
+    return worst;
+}
+}  // namespace
+
+extern "C" int islbm_selftest_collision(double out[4])
+{
+    if (!out) return fail(ISLBM_ERR_ARG, "null argument");
+    uint64_t st = 20261203;
+    out[0] = collision_forms_gap<false, double>(0, st);
+    out[1] = collision_forms_gap<false, double>(1, st);
+    out[2] = collision_forms_gap<true, float>(0, st);
+    out[3] = collision_forms_gap<true, float>(1, st);
+    return ISLBM_OK;
+}

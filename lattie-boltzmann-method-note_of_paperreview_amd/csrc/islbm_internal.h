@@ -77,6 +77,7 @@ struct StepLaunch {
     const float* scale;   // per-direction 16-bit scale (device), or null
 };
 hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L);
+hipError_t launch_stream_copy(hipStream_t st, const void* src, void* dst, long long bytes);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
 
 hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw,

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_collide_pernode(StepArgs<ST> a)
 
     C tau_eff = (C)a.tau;
     C rho, ux, uy, uz;
-    collide_node<CD::kDev>(F, (C)*a.rho_modify, (C)a.dt, (C)(*a.force), a.les ? a.t.les_delta[(long long)k * ny + j] : C(0),
+    collide_pairs<CD::kDev>(F, (C)*a.rho_modify, (C)a.dt, (C)(*a.force), a.les ? a.t.les_delta[(long long)k * ny + j] : C(0),
                            a.les, (C)a.tau, (C)a.cs, rho, ux, uy, uz, tau_eff);
 
 #pragma unroll
@@ -255,6 +255,22 @@ hipError_t launch_unique_to_padded(hipStream_t st, const PtrSet& uni, const PtrS
     const long long tot = (long long)(nx + 7) * (ny + 7) * nz * narr;
     hipLaunchKernelGGL(k_unique_to_padded, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, uni, pad, nx, ny,
                        nz, narr);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Plain streaming copy (float4), the bandwidth reference and counter calibration kernel
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst, long long n4)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride) dst[t] = src[t];
+}
+
+hipError_t launch_stream_copy(hipStream_t st, const void* src, void* dst, long long bytes)
+{
+    const long long n4 = bytes / 16;
+    hipLaunchKernelGGL(k_stream_copy, dim3(256 * 16), dim3(256), 0, st, (const float4*)src, (float4*)dst, n4);
     return hipGetLastError();
 }
 

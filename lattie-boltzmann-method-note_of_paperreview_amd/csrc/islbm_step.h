@@ -154,7 +154,7 @@ template <class R, class C>
 __device__ inline C ip_xi_y(const R& f, long long base, const int* yo, long long plane, const C* wy, const C* wz)
 {
     C r = C(0);
-#pragma unroll
+#pragma unroll 1
     for (int n = 0; n < 7; ++n) {
         C in = f(base + yo[n]) * wz[0];
 #pragma unroll
@@ -169,7 +169,7 @@ template <class R, class C>
 __device__ inline C ip_y_xi(const R& f, long long base, const int* yo, long long plane, const C* wy, const C* wz)
 {
     C r = C(0);
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < 7; ++m) {
         C in = f(base + m * plane + yo[0]) * wy[0];
 #pragma unroll
@@ -184,7 +184,7 @@ template <class R, class C>
 __device__ inline C ip_x_xi(const R& f, long long base, const int* xo, long long plane, const C* wx, const C* wz)
 {
     C r = C(0);
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < 7; ++m) {
         C in = f(base + m * plane + xo[0]) * wx[0];
 #pragma unroll
@@ -200,7 +200,7 @@ __device__ inline C ip_x_y_xi(const R& f, long long base, const int* xo, const i
                               const C* wy, const C* wz)
 {
     C r = C(0);
-#pragma unroll
+#pragma unroll 1
     for (int m = 0; m < 7; ++m) {
         C mid = C(0);
 #pragma unroll
@@ -288,7 +288,7 @@ __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long 
 // LES: tau* = tau + 3 nu_t/dt, nu_t = (Cs D)^2 |S|, |S| from Q = Pi_neq:Pi_neq in closed form.
 // ------------------------------------------------------------------------------------
 template <bool kDev, class C>
-__device__ inline void collide_node(C (&F)[NQ], C rho_modify, C dt, C force, C delta, int les, C tau, C cs, C& rho_out,
+__host__ __device__ inline void collide_node(C (&F)[NQ], C rho_modify, C dt, C force, C delta, int les, C tau, C cs, C& rho_out,
                                     C& ux_out, C& uy_out, C& uz_out, C& tau_eff)
 {
     F[0] = F[0] + rho_modify;
@@ -377,6 +377,128 @@ __device__ inline void collide_node(C (&F)[NQ], C rho_modify, C dt, C force, C d
         }
         F[q] = v;
     }
+    rho_out = rho;
+    ux_out = ux;
+    uy_out = uy;
+    uz_out = uz;
+}
+
+
+// ------------------------------------------------------------------------------------
+// The same collision evaluated through the D3Q19 opposite-pair structure.  With
+// s = f_q + f_qbar and a = f_q - f_qbar for the 9 pairs (1,2) (3,4) (5,6) (7,10) (8,9)
+// (11,14) (12,13) (15,18) (16,17), the even moments of (f - f_eq) are sums of s, the odd
+// ones sums of a, and the update of a pair is E +- O with E the even-row and O the
+// odd-row back-transform.  Algebraically identical to collide_node; about half the
+// arithmetic.  The body force enters the odd part of the y-moving pairs.
+// ------------------------------------------------------------------------------------
+template <bool kDev, class C>
+__host__ __device__ inline void collide_pairs(C (&F)[NQ], C rho_modify, C dt, C force, C delta, int les, C tau, C cs,
+                                              C& rho_out, C& ux_out, C& uy_out, C& uz_out, C& tau_eff)
+{
+    F[0] = F[0] + rho_modify;
+    C sum = F[0];
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) sum = sum + F[q];
+    const C rho = kDev ? C(1) + sum : sum;
+    const C irho = C(1) / rho;
+    const C ux = (F[1] + F[7] + F[9] + F[11] + F[13] - (F[2] + F[8] + F[10] + F[12] + F[14])) * irho;
+    const C uy = (F[3] + F[7] + F[8] + F[15] + F[17] - (F[4] + F[9] + F[10] + F[16] + F[18])) * irho;
+    const C uz = (F[5] + F[11] + F[12] + F[15] + F[16] - (F[6] + F[13] + F[14] + F[17] + F[18])) * irho;
+    const C udot = ux * ux + uy * uy + uz * uz;
+    // f_eq = w rho (1 + 3 eu + 4.5 eu^2 - 1.5 u^2); deviation form subtracts w
+    const C base = kDev ? sum - rho * C(1.5) * udot : rho * (C(1) - C(1.5) * udot);  // rho(1-1.5u^2) [- 1]
+    const C r3 = rho * C(3), r45 = rho * C(4.5);
+    // pair (q, qbar) with e.u = p:  s_eq = 2 w (base + 4.5 rho p^2),  a_eq = 2 w 3 rho p
+    auto pr = [&](int q, int qb, C p, C w2, C& s, C& a) {
+        s = F[q] + F[qb] - w2 * (base + r45 * p * p);
+        a = F[q] - F[qb] - w2 * (r3 * p);
+    };
+    const C w18 = C(2.0 / 18.0), w36 = C(2.0 / 36.0);
+    C s1, a1, s3, a3, s5, a5, s7, a7, s8, a8, s11, a11, s12, a12, s15, a15, s16, a16;
+    pr(1, 2, ux, w18, s1, a1);
+    pr(3, 4, uy, w18, s3, a3);
+    pr(5, 6, uz, w18, s5, a5);
+    pr(7, 10, ux + uy, w36, s7, a7);
+    pr(8, 9, uy - ux, w36, s8, a8);
+    pr(11, 14, ux + uz, w36, s11, a11);
+    pr(12, 13, uz - ux, w36, s12, a12);
+    pr(15, 18, uy + uz, w36, s15, a15);
+    pr(16, 17, uz - uy, w36, s16, a16);
+    const C d0 = F[0] - C(1.0 / 3.0) * base;
+
+    // even moments of f - f_eq
+    const C sf = s1 + s3 + s5, se = s7 + s8 + s11 + s12 + s15 + s16;
+    const C exy = s7 + s8, exz = s11 + s12, eyz = s15 + s16;
+    const C m1 = C(-30) * d0 - C(11) * sf + C(8) * se;
+    const C m2 = C(12) * d0 - C(4) * sf + se;
+    const C m9 = C(2) * s1 - s3 - s5 + exy + exz - C(2) * eyz;
+    const C m10 = C(-4) * s1 + C(2) * (s3 + s5) + exy + exz - C(2) * eyz;
+    const C m11 = s3 - s5 + exy - exz;
+    const C m12 = C(-2) * s3 + C(2) * s5 + exy - exz;
+    const C m13 = s7 - s8, m14 = s15 - s16, m15 = s11 - s12;
+    // odd moments
+    const C m4 = C(-4) * a1 + a7 - a8 + a11 - a12;
+    const C m6 = C(-4) * a3 + a7 + a8 + a15 - a16;
+    const C m8 = C(-4) * a5 + a11 + a12 + a15 + a16;
+    const C m16 = a7 - a8 - a11 + a12;
+    const C m17 = -a7 - a8 + a15 - a16;
+    const C m18 = a11 + a12 - a15 - a16;
+
+    C s_nu = C(1) / tau;
+    tau_eff = tau;
+    if (les) {
+        const C pxx = s1 + exy + exz, pyy = s3 + exy + eyz, pzz = s5 + exz + eyz;
+        const C Q = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (m13 * m13 + m14 * m14 + m15 * m15);
+        const C Cd = cs * cs * delta * delta;
+        const C b = tau * dt;
+        const C S = (sqrt(b * b + C(18.0 * 1.4142135623730951) * Cd * sqrt(Q) * irho) - b) / (C(6) * Cd);
+        tau_eff = tau + C(3) * Cd * S / dt;
+        s_nu = C(1) / tau_eff;
+    }
+    // r_a = s_a m_a / |M_a|^2
+    const C r1 = m1 * C(1.19 / 2394.0), r2 = m2 * C(1.4 / 252.0);
+    const C r4 = m4 * C(1.2 / 40.0), r6 = m6 * C(1.2 / 40.0), r8 = m8 * C(1.2 / 40.0);
+    const C r9 = m9 * (s_nu / C(36)), r10 = m10 * C(1.4 / 72.0);
+    const C r11 = m11 * (s_nu / C(12)), r12 = m12 * C(1.4 / 24.0);
+    const C r13 = m13 * (s_nu / C(4)), r14 = m14 * (s_nu / C(4)), r15 = m15 * (s_nu / C(4));
+    const C r16 = m16 * C(1.5 / 8.0), r17 = m17 * C(1.5 / 8.0), r18 = m18 * C(1.5 / 8.0);
+
+    // back-transform: even part E per pair class, odd part O per pair
+    const C fe = C(-11) * r1 - C(4) * r2;   // faces
+    const C ee = C(8) * r1 + r2;            // edges
+    const C E0 = C(-30) * r1 + C(12) * r2;
+    const C E1 = fe + C(2) * r9 - C(4) * r10;
+    const C E3 = fe - r9 + C(2) * r10 + r11 - C(2) * r12;
+    const C E5 = fe - r9 + C(2) * r10 - r11 + C(2) * r12;
+    const C exy_ = ee + r9 + r10 + r11 + r12;
+    const C exz_ = ee + r9 + r10 - r11 - r12;
+    const C eyz_ = ee - C(2) * r9 - C(2) * r10;
+    const C fdt = force * dt;
+    const C f6 = fdt * C(1.0 / 6), f12 = fdt * C(1.0 / 12);
+    const C O1 = C(-4) * r4;
+    const C O3 = C(-4) * r6 - f6;
+    const C O5 = C(-4) * r8;
+    const C O7 = r4 + r6 + r16 - r17 - f12;
+    const C O8 = -r4 + r6 - r16 - r17 - f12;
+    const C O11 = r4 + r8 - r16 + r18;
+    const C O12 = -r4 + r8 + r16 + r18;
+    const C O15 = r6 + r8 + r17 - r18 - f12;
+    const C O16 = -r6 + r8 - r17 - r18 + f12;
+    F[0] = F[0] - E0;
+    auto upd = [&](int q, int qb, C E, C O) {
+        F[q] = F[q] - (E + O);
+        F[qb] = F[qb] - (E - O);
+    };
+    upd(1, 2, E1, O1);
+    upd(3, 4, E3, O3);
+    upd(5, 6, E5, O5);
+    upd(7, 10, exy_ + r13, O7);
+    upd(8, 9, exy_ - r13, O8);
+    upd(11, 14, exz_ + r15, O11);
+    upd(12, 13, exz_ - r15, O12);
+    upd(15, 18, eyz_ + r14, O15);
+    upd(16, 17, eyz_ - r14, O16);
     rho_out = rho;
     ux_out = ux;
     uy_out = uy;

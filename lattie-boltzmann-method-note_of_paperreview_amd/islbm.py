@@ -27,6 +27,7 @@ HEADER_SYMBOLS = [
     "islbm_step", "islbm_sync", "islbm_modify_forcing", "islbm_get_force", "islbm_set_force",
     "islbm_set_rho_modify", "islbm_get_flux_sum", "islbm_scalars", "islbm_step_count",
     "islbm_kernel_in_use", "islbm_get_table", "islbm_time_steps", "islbm_last_error",
+    "islbm_selftest_collision", "islbm_copy_bandwidth",
     "islbm_ref_init", "islbm_ref_shutdown", "islbm_ref_device_arrays",
     "Launch_CollisionStreaming", "Launch_ModifyForcingTerm",
 ]
@@ -86,6 +87,8 @@ def lib():
     L.islbm_get_table.argtypes = [vp, ctypes.c_char_p, dp, i32]
     L.islbm_time_steps.argtypes = [vp, i32, dp]
     L.islbm_last_error.restype = ctypes.c_char_p
+    L.islbm_selftest_collision.argtypes = [dp]
+    L.islbm_copy_bandwidth.argtypes = [i32, ctypes.c_size_t, i32, dp]
     L.islbm_ref_init.argtypes = [pp]
     L.islbm_ref_device_arrays.argtypes = [ctypes.POINTER(vp)] * 6
     L.Launch_CollisionStreaming.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp)]
@@ -238,6 +241,13 @@ class Solver:
         o = np.zeros(2)
         _check(lib().islbm_time_steps(self.h, int(n), _dp(o)), "islbm_time_steps")
         return float(o[0]), float(o[1])
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20, device: int = 0) -> float:
+    """GB/s of a plain device copy kernel (bytes read + written)."""
+    v = ctypes.c_double()
+    _check(lib().islbm_copy_bandwidth(int(device), int(nbytes), int(iters), ctypes.byref(v)), "islbm_copy_bandwidth")
+    return v.value
 
 
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:

@@ -69,3 +69,13 @@ def test_reference_entry_point_without_init_records_error(pkg):
     L.islbm_ref_shutdown()
     L.Launch_CollisionStreaming(arr, arr)
     assert b"islbm_ref_init" in L.islbm_last_error()
+
+
+def test_collision_forms_agree_on_host(pkg):
+    """The opposite-pair MRT used by the kernels equals the row-by-row form of
+    MRT_Process.h to round-off (host-only check inside the library)."""
+    import numpy as np
+    out = np.zeros(4)
+    assert pkg.lib().islbm_selftest_collision(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    assert out[0] < 1e-15 and out[1] < 1e-15, out
+    assert out[2] < 1e-7 and out[3] < 5e-7, out

@@ -30,8 +30,11 @@ HBM_PEAK_GBS = 8000.0
 
 GRID = dict(nx=128, ny=256, nz=128)
 PHYSICS = {
-    # the reference's shipped operating point (tau 0.6833, Re 50) with the Smagorinsky
-    # relaxation evaluated (Cs 0.1): the LES cost is in the step, the state stays bounded
+    # Re 50 at tau 1.0 with the Smagorinsky relaxation evaluated (Cs 0.1): the LES cost is in
+    # the step and the flow stays bounded on this grid
+    "stable-les": dict(tau=1.0, re=50.0, u_ref=0.0, cs_smag=0.1, mesh_mode=0),
+    # the reference's shipped operating point with Cs 0.1; on this grid the reference scheme
+    # diverges within ~100 steps (tau 0.6833 and 0.8 both do, at Re 10 and 50)
     "reference-les": dict(tau=0.6833, re=50.0, u_ref=0.0, cs_smag=0.1, mesh_mode=0),
     # the reference's shipped operating point, plain MRT
     "reference": dict(tau=0.6833, re=50.0, u_ref=0.0, cs_smag=0.0, mesh_mode=0),
@@ -60,7 +63,7 @@ def cpu_baseline(physics: dict, steps: int):
     pyoracle.build()
     pyoracle.threads(threads)
     o = pyoracle.Oracle(pyoracle.default_params(**GRID, **{k: v for k, v in physics.items()}))
-    o.init_equilibrium(u0=(0.0, o.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    o.init_equilibrium(u0=(0.0, 0.0, 0.0), amp=1e-5, seed=20261203)
     o.step(1)  # first touch
     t0 = time.perf_counter()
     o.step(steps)
@@ -78,7 +81,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
     ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled"])
-    ap.add_argument("--physics", default="reference-les", choices=sorted(PHYSICS))
+    ap.add_argument("--physics", default="stable-les", choices=sorted(PHYSICS))
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -92,7 +95,9 @@ def main() -> int:
     phys = PHYSICS[args.physics]
     p = pkg.default_params(**GRID, **phys, precision=args.precision, kernel=kern)
     s = pkg.Solver(p)
-    s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    # the reference's start (InitialUsingDftFunc): fluid at rest, driven by the body force;
+    # a 1e-5 seeded velocity perturbation breaks the exact symmetry
+    s.init_equilibrium(u0=(0.0, 0.0, 0.0), amp=1e-5, seed=20261203)
     n = GRID["nx"] * GRID["ny"] * GRID["nz"]
     print(f"[bench] {args.precision} {args.physics} kernel={s.kernel_in_use()} warmup {args.warmup}",
           file=sys.stderr, flush=True)

@@ -5,6 +5,8 @@
 
 #include "islbm_internal.h"
 
+#include <type_traits>
+
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -216,6 +218,45 @@ __device__ inline C ip_x_y_xi(const R& f, long long base, const int* xo, const i
     return r;
 }
 
+// The two BFL gathers of the wall treatment, in the same summation order as ip_y_xi and
+// ip_x_y_xi but with every loop but the innermost rolled: they run on the first two
+// levels only, and fully unrolled their 49 addresses per xi level would set the register
+// allocation of the whole kernel they are inlined into.
+template <class R, class C>
+__device__ inline C wall_y_xi(const R& f, long long base, const int* yo, long long plane, const C* wy, const C* wz)
+{
+    C r = C(0);
+#pragma unroll 1
+    for (int m = 0; m < 7; ++m) {
+        C in = C(0);
+#pragma unroll 1
+        for (int n = 0; n < 7; ++n) in = (n == 0) ? f(base + m * plane + yo[0]) * wy[0] : in + f(base + m * plane + yo[n]) * wy[n];
+        r = (m == 0) ? in * wz[0] : r + in * wz[m];
+    }
+    return r;
+}
+
+template <class R, class C>
+__device__ inline C wall_x_y_xi(const R& f, long long base, const int* xo, const int* yo, long long plane, const C* wx,
+                                const C* wy, const C* wz)
+{
+    C r = C(0);
+#pragma unroll 1
+    for (int m = 0; m < 7; ++m) {
+        C mid = C(0);
+#pragma unroll 1
+        for (int n = 0; n < 7; ++n) {
+            const long long b = base + m * plane + yo[n];
+            C in = f(b + xo[0]) * wx[0];
+#pragma unroll
+            for (int l = 1; l < 7; ++l) in = in + f(b + xo[l]) * wx[l];
+            mid = (n == 0) ? in * wy[0] : mid + in * wy[n];
+        }
+        r = (m == 0) ? mid * wz[0] : r + mid * wz[m];
+    }
+    return r;
+}
+
 // ------------------------------------------------------------------------------------
 // Walls.  Flat-wall half-way bounce-back with the stored (previous post-collision)
 // populations of the node (evolution.h:482-496), then BFL on the first two layers above
@@ -249,9 +290,9 @@ __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long 
             F[7] = c1 * at(10) + c2 * at(7);
             F[8] = c1 * at(9) + c2 * at(8);
         } else if (f3 == 2) {
-            F[3] = ip_y_xi(pop(4), (long long)i, yo, plane, W(BY378), W(BXI378));
-            F[7] = ip_x_y_xi(pop(10), 0LL, xo, yo, plane, W(BX37), W(BY378), W(BXI378));
-            F[8] = ip_x_y_xi(pop(9), 0LL, xo, yo, plane, W(BX38), W(BY378), W(BXI378));
+            F[3] = wall_y_xi(pop(4), (long long)i, yo, plane, W(BY378), W(BXI378));
+            F[7] = wall_x_y_xi(pop(10), 0LL, xo, yo, plane, W(BX37), W(BY378), W(BXI378));
+            F[8] = wall_x_y_xi(pop(9), 0LL, xo, yo, plane, W(BX38), W(BY378), W(BXI378));
         }
         if (f4 == 1) {
             const C q = a.t.bq[B4 * nb + ib];
@@ -260,21 +301,21 @@ __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long 
             F[9] = c1 * at(8) + c2 * at(9);
             F[10] = c1 * at(7) + c2 * at(10);
         } else if (f4 == 2) {
-            F[4] = ip_y_xi(pop(3), (long long)i, yo, plane, W(BY4910), W(BXI4910));
-            F[9] = ip_x_y_xi(pop(8), 0LL, xo, yo, plane, W(BX49), W(BY4910), W(BXI4910));
-            F[10] = ip_x_y_xi(pop(7), 0LL, xo, yo, plane, W(BX410), W(BY4910), W(BXI4910));
+            F[4] = wall_y_xi(pop(3), (long long)i, yo, plane, W(BY4910), W(BXI4910));
+            F[9] = wall_x_y_xi(pop(8), 0LL, xo, yo, plane, W(BX49), W(BY4910), W(BXI4910));
+            F[10] = wall_x_y_xi(pop(7), 0LL, xo, yo, plane, W(BX410), W(BY4910), W(BXI4910));
         }
         if (f15 == 1) {
             const C q = a.t.bq[B15 * nb + ib];
             F[15] = (C(1) / (C(2) * q)) * at(18) + ((C(2) * q - C(1)) / (C(2) * q)) * at(15);
         } else if (f15 == 2) {
-            F[15] = ip_y_xi(pop(18), (long long)i, yo, plane, W(BY15), W(BXI15));
+            F[15] = wall_y_xi(pop(18), (long long)i, yo, plane, W(BY15), W(BXI15));
         }
         if (f16 == 1) {
             const C q = a.t.bq[B16 * nb + ib];
             F[16] = (C(1) / (C(2) * q)) * at(17) + ((C(2) * q - C(1)) / (C(2) * q)) * at(16);
         } else if (f16 == 2) {
-            F[16] = ip_y_xi(pop(17), (long long)i, yo, plane, W(BY16), W(BXI16));
+            F[16] = wall_y_xi(pop(17), (long long)i, yo, plane, W(BY16), W(BXI16));
         }
     }
 }
@@ -392,16 +433,45 @@ __host__ __device__ inline void collide_node(C (&F)[NQ], C rho_modify, C dt, C f
 // odd-row back-transform.  Algebraically identical to collide_node; about half the
 // arithmetic.  The body force enters the odd part of the y-moving pairs.
 // ------------------------------------------------------------------------------------
+// FP32 on the device uses the hardware reciprocal and square root (about 1 ulp); FP64
+// and the host keep the correctly rounded division and square root.
+template <class C>
+__host__ __device__ inline constexpr bool use_fast_math()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return std::is_same_v<C, float>;
+#else
+    return false;
+#endif
+}
+template <class C>
+__host__ __device__ inline C rcp_(C x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same_v<C, float>) return __builtin_amdgcn_rcpf(x);
+#endif
+    return C(1) / x;
+}
+template <class C>
+__host__ __device__ inline C sqrt_(C x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same_v<C, float>) return __builtin_amdgcn_sqrtf(x);
+#endif
+    return sqrt(x);
+}
+
 template <bool kDev, class C>
 __host__ __device__ inline void collide_pairs(C (&F)[NQ], C rho_modify, C dt, C force, C delta, int les, C tau, C cs,
                                               C& rho_out, C& ux_out, C& uy_out, C& uz_out, C& tau_eff)
 {
+    constexpr bool kFast = use_fast_math<C>();
     F[0] = F[0] + rho_modify;
     C sum = F[0];
 #pragma unroll
     for (int q = 1; q < NQ; ++q) sum = sum + F[q];
     const C rho = kDev ? C(1) + sum : sum;
-    const C irho = C(1) / rho;
+    const C irho = rcp_(rho);
     const C ux = (F[1] + F[7] + F[9] + F[11] + F[13] - (F[2] + F[8] + F[10] + F[12] + F[14])) * irho;
     const C uy = (F[3] + F[7] + F[8] + F[15] + F[17] - (F[4] + F[9] + F[10] + F[16] + F[18])) * irho;
     const C uz = (F[5] + F[11] + F[12] + F[15] + F[16] - (F[6] + F[13] + F[14] + F[17] + F[18])) * irho;
@@ -445,16 +515,21 @@ __host__ __device__ inline void collide_pairs(C (&F)[NQ], C rho_modify, C dt, C 
     const C m17 = -a7 - a8 + a15 - a16;
     const C m18 = a11 + a12 - a15 - a16;
 
-    C s_nu = C(1) / tau;
+    C s_nu = rcp_(tau);
     tau_eff = tau;
     if (les) {
         const C pxx = s1 + exy + exz, pyy = s3 + exy + eyz, pzz = s5 + exz + eyz;
         const C Q = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (m13 * m13 + m14 * m14 + m15 * m15);
         const C Cd = cs * cs * delta * delta;
         const C b = tau * dt;
-        const C S = (sqrt(b * b + C(18.0 * 1.4142135623730951) * Cd * sqrt(Q) * irho) - b) / (C(6) * Cd);
-        tau_eff = tau + C(3) * Cd * S / dt;
-        s_nu = C(1) / tau_eff;
+        if constexpr (kFast) {
+            const C S = (sqrt_(b * b + C(18.0 * 1.4142135623730951) * Cd * sqrt_(Q) * irho) - b) * rcp_(C(6) * Cd);
+            tau_eff = tau + C(3) * Cd * S * rcp_(dt);
+        } else {
+            const C S = (sqrt(b * b + C(18.0 * 1.4142135623730951) * Cd * sqrt(Q) * irho) - b) / (C(6) * Cd);
+            tau_eff = tau + C(3) * Cd * S / dt;
+        }
+        s_nu = rcp_(tau_eff);
     }
     // r_a = s_a m_a / |M_a|^2
     const C r1 = m1 * C(1.19 / 2394.0), r2 = m2 * C(1.4 / 252.0);

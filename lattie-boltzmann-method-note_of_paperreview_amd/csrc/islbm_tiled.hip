@@ -1,29 +1,36 @@
 // islbm_tiled.hip -- column-marching, LDS-staged, sum-factorised stream-collide kernel for
 // FP32 and 16-bit population storage (FP32 arithmetic, deviation form f - w_q).
 //
-// Work decomposition.  A workgroup of 256 lanes owns a 64 (x) by 2 (y) tile of node
-// columns and marches up a chunk of z levels.  Its four waves form two groups of two
-// (one wave per y row in each group); the groups split the 16 directions that need a
-// 7-level xi stencil, so a lane keeps a ring of 8 directions x 7 levels in registers:
-//     group A: 5 6 7 8 9 10 11 12        group B: 3 4 13 14 15 16 17 18
-// For every z level the 14 populations with horizontal neighbours are staged into LDS
-// (double-buffered, with a 3-node halo where the direction needs one: 7-10 in x and y,
-// 3, 4, 15-18 in y, 11-14 in x; 5 and 6 are read at the lane's own node).  The separable
-// x and y Lagrange filters run from LDS (the y pass of 7-10 is shared through LDS) and the
-// filtered values enter the ring; once the ring covers a target level's xi stencil, each
-// group applies the xi weights of the target node to its 8 directions (the last step of
-// the interpolated streaming of interpolationHillISLBM.h).  Targets alternate between the
-// groups: the other group hands over its 8 values through LDS, and the owner reads
-// directions 0, 1, 2 (no z dependence), applies the walls, collides (collide_pairs,
-// evolution.h:597-679) and stores.
+// Work decomposition.  A workgroup owns a 64 (x) by 2 (y) tile of node columns and
+// marches up a chunk of z levels.  It has six waves in two roles:
+//   ring waves (4): two direction groups x two y rows; a lane keeps a ring of its group's
+//       8 directions x 7 levels in registers
+//           group A: 5 6 7 8 9 10 11 12        group B: 3 4 13 14 15 16 17 18
+//   collision waves (2): one per y row; they take the 16 interpolated populations from
+//       LDS, read 0, 1, 2 themselves, apply the walls, collide (collide_pairs,
+//       evolution.h:597-679) and store.
+// The roles run separate loops with one workgroup barrier per step, so each role's
+// registers are allocated for that role alone and the collision of one target overlaps
+// the ring work of the next.
 //
-// FP32 is staged with direct global-to-LDS loads (global_load_lds_dword), one 64-lane row
-// (plus a 6-lane halo tail) per load, so no registers hold the prefetch; 16-bit storage
-// is staged with global_load_lds_ushort (one zero-extended dword per lane) and decoded in
-// place.  Blocks are mapped to tiles so that each XCD (blocks dispatched round-robin over
-// 8 XCDs) works on a contiguous slab of tiles, keeping the y-halo rows of neighbouring
-// tiles in one L2.  The uniform mesh in x and y makes the x and y weights identical for
-// every column and row (to FP64 round-off); the kernel reads them as wave-uniform values.
+// Per z level the ring waves stage the 14 populations with horizontal neighbours into LDS
+// (double-buffered, with a 3-node halo where the direction needs one: 7-10 in x and y,
+// 3, 4, 15-18 in y, 11-14 in x; 5 and 6 are read at the lane's own node), run the
+// separable x and y Lagrange filters from LDS (group A runs the y pass of 7-10 for its own
+// row in a wave-private LDS area) and append the filtered values to the ring.  Once the
+// ring covers a target level's xi stencil, each ring wave applies the xi weights of the
+// target node to its 8 directions (the last step of the interpolated streaming of
+// interpolationHillISLBM.h) and hands the results to the collision waves through LDS.
+//
+// Staging uses buffer loads straight into LDS (buffer_load_dword/ushort ... lds), one
+// 64-lane row per load, so no registers hold the prefetch; 16-bit storage lands as one
+// zero-extended dword per lane and is decoded in place by the wave that loaded it.
+// Blocks are mapped to tiles so that each XCD (blocks dispatched round-robin over 8 XCDs)
+// works on a contiguous slab of tiles, keeping the y-halo rows of neighbouring tiles in
+// one L2.  The uniform mesh in x and y makes the x and y weights identical for every
+// column and row (to FP64 round-off); the kernel reads them as wave-uniform values.
+#include <type_traits>
+
 #include "islbm_internal.h"
 #include "islbm_step.h"
 
@@ -31,7 +38,10 @@ namespace islbm {
 
 namespace {
 
-constexpr int TX = 64, TY = 2, NTH = 256;
+constexpr int TX = 64, TY = 2;
+constexpr int NRW = 2 * TY;  // ring waves
+constexpr int NCW = TY;      // collision waves
+constexpr int NTH = 64 * (NRW + NCW);
 constexpr int HX = TX + 6, HY = TY + 6;
 constexpr int XY_SZ = HY * HX;  // 560: tile of a diagonal direction (x and y halo)
 constexpr int Y_SZ = HY * TX;   // 512: y halo only
@@ -40,16 +50,17 @@ constexpr int OFF_XY = 0;
 constexpr int OFF_Y = OFF_XY + 4 * XY_SZ;
 constexpr int OFF_X = OFF_Y + 6 * Y_SZ;
 constexpr int STAGE = OFF_X + 4 * X_SZ;  // 5872 staged values per level
-constexpr int YP_SZ = TY * HX;           // 140: y pass of a diagonal direction
-constexpr int OFF_YP = 2 * STAGE;
-constexpr int OFF_XF = OFF_YP + 4 * YP_SZ;       // hand-over of 8 directions x 128 columns
-constexpr int LDS_FLOATS = OFF_XF + 8 * TX * TY;  // 13328 floats = 52 KB
+constexpr int OFF_YP = 2 * STAGE;        // group A: y pass of 7-10 for its row, 4 x HX per wave
+constexpr int HO_SZ = 16 * TX * TY;      // hand-over: directions 3-18 x 128 columns
+constexpr int OFF_HO = OFF_YP + TY * 4 * HX;
+constexpr int LDS_FLOATS = OFF_HO + 2 * HO_SZ;  // 16400 floats = 64 KB: two workgroups per CU
+static_assert(TY == 2, "the fetch split below assumes two rows per tile");
+__host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
 
-__device__ constexpr int kXY[4] = {7, 8, 9, 10};
-__device__ constexpr int kY[6] = {3, 4, 15, 16, 17, 18};
-__device__ constexpr int kX[4] = {11, 12, 13, 14};
-__device__ constexpr int kGA[8] = {5, 6, 7, 8, 9, 10, 11, 12};
-__device__ constexpr int kGB[8] = {3, 4, 13, 14, 15, 16, 17, 18};
+// staged direction of each y-halo LDS region
+__host__ __device__ constexpr int qy(int d) { return d < 2 ? 3 + d : 13 + d; }  // 3 4 15 16 17 18
+// the 8 directions of ring group G
+__host__ __device__ constexpr int gq(int G, int d) { return G == 0 ? 5 + d : (d < 2 ? 3 + d : 11 + d); }
 __host__ __device__ constexpr int xsel(int q) { return cx(q) > 0 ? 0 : 1; }  // 0: X0 (x - dt), 1: X2
 __host__ __device__ constexpr int ysel(int q) { return cy(q) > 0 ? 0 : 1; }
 // xi family of each interpolated direction (initialization.h:235-242, evolution.h:457-475)
@@ -69,9 +80,10 @@ TiledGeom tiled_geom(int nx, int ny, int nz)
     t.ntx = (nx + TX - 1) / TX;
     t.nty = (ny + TY - 1) / TY;
     t.ntiles = t.ntx * t.nty;
-    // about three resident workgroups per CU on 256 CUs, z chunks of at least 8 levels
-    int want = (768 + t.ntiles - 1) / t.ntiles;
-    int maxc = nz / 8 > 0 ? nz / 8 : 1;
+    // two resident workgroups per CU on 256 CUs, z chunks of at least 16 levels (each
+    // chunk re-reads a 6-level xi halo)
+    int want = (512 + t.ntiles - 1) / t.ntiles;
+    int maxc = nz / 16 > 0 ? nz / 16 : 1;
     t.nchunks = want < 1 ? 1 : (want > maxc ? maxc : want);
     t.kc = (nz + t.nchunks - 1) / t.nchunks;
     t.nchunks = (nz + t.kc - 1) / t.kc;
@@ -82,6 +94,75 @@ TiledGeom tiled_geom(int nx, int ny, int nz)
 
 __device__ inline int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Buffer resource over one population (or field) array: 32-bit byte offsets from its start.
+__device__ inline __amdgpu_buffer_rsrc_t rsrc_of(const void* base, int nbytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nbytes, 0x00020000);
+}
+
+// One population value through a buffer resource, decoded to / encoded from the FP32
+// deviation form.
+template <Store ST>
+struct BufIO {
+    using CD = Codec<ST>;
+    using raw = typename CD::raw;
+    __device__ static float ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float isc)
+    {
+        if constexpr (sizeof(raw) == 4) {
+            return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+        } else {
+            const unsigned short code = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+            const raw v = __builtin_bit_cast(raw, code);
+            return CD::ld(&v, 0, isc);
+        }
+    }
+    __device__ static void st(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float x, float sc)
+    {
+        raw v;
+        CD::st(&v, 0, x, sc);
+        if constexpr (sizeof(raw) == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)voff, (int)soff, 0);
+    }
+};
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations
+// (lgkmcnt(0)) and meets the other waves.  __syncthreads() adds a workgroup release fence,
+// which on this target waits for every outstanding global load as well (vmcnt(0)),
+// including the next level's LDS prefetch, and would serialise the prefetch with the
+// filters.  The kernel shares no global memory between waves, so the fence is not needed;
+// the empty asm statements keep the compiler from moving LDS accesses across the barrier.
+__device__ inline void lds_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt not waited for
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Step schedule, evaluated identically by every wave of the workgroup.  A step appends at
+// most one level to the ring (adv) and hands at most one target level over (ho >= 0);
+// targets k0..k1 are handed over in order as soon as the ring holds their xi stencil
+// (levels base..base+6, base = xi_stencil_start).  Interior targets take one step each;
+// the clamped stencils of the first and last four levels of the column hand over up to
+// three more targets from the same ring in steps that append nothing.
+struct Sched {
+    int top;  // newest level in the ring
+    int t;    // next target level
+};
+__device__ inline bool sched_step(Sched& sc, int k1, int nz, bool& adv, int& ho)
+{
+    adv = false;
+    ho = -1;
+    if (sc.t > k1) return false;
+    const int need = xi_stencil_start(sc.t, nz) + 6;
+    adv = sc.top < need;
+    if (adv) ++sc.top;
+    if (sc.top == need) ho = sc.t++;
+    return true;
+}
+
 }  // namespace
 
 template <Store ST>
@@ -90,7 +171,9 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
 {
     using CD = Codec<ST>;
     using raw = typename CD::raw;
+    using IO = BufIO<ST>;
     constexpr bool k16 = (ST != Store::F32);
+    constexpr unsigned SZ = sizeof(raw);
     __shared__ float lds[LDS_FLOATS];
 
     // XCD-aware work mapping: block b runs on XCD b % 8
@@ -104,287 +187,342 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long long plane = (long long)nx * ny;
     const long long N = a.n;
-    const int t = threadIdx.x;
-    const int lane = t % 64;
-    const int wave = __builtin_amdgcn_readfirstlane(t / 64);
-    const int grp = wave / TY;  // 0: A, 1: B
-    const int ty = wave % TY;
-    const int tx = lane;
-    const int col = ty * TX + tx;  // column within the tile
-    const int i = i0 + tx;
-    const int j = j0 + ty;
-    const bool active = (i < nx) && (j < ny);
-    const int iw = i < nx ? i : i % nx;
-    const int jw = j < ny ? j : j % ny;
+    const int nbytes = (int)(SZ * (unsigned)N);  // one population array (tiled_supported: < 2^31)
+    const int lane = threadIdx.x % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int k0 = chunk * tg.kc;
     const int k1 = min(nz, k0 + tg.kc) - 1;
     const int L0 = xi_stencil_start(k0, nz);
     const int L1 = xi_stencil_start(k1, nz) + 6;
-    const unsigned own = (unsigned)(jw * nx + iw);
+    const int tx = lane;
+    const int i = i0 + tx;
+    const int iw = i < nx ? i : i % nx;
+    auto popr = [&](int q) { return rsrc_of(a.fo + (long long)q * N, nbytes); };
 
-    // this lane's columns in the staged rows: halo rows start at i0 - 3 (64 lanes + a
-    // 6-lane tail), plain rows at i0
-    const unsigned xh = (unsigned)wrap_index(i0 - 3 + lane, nx);
-    const unsigned xt = (unsigned)wrap_index(i0 - 3 + 64 + (lane < 6 ? lane : 0), nx);
-    const unsigned xn = (unsigned)iw;
+    if (wave < NRW) {
+        // ------------------------------------------------------------------ ring waves
+        const int w = wave;  // fetch share
+        const int ty = w % TY;
+        const int col = ty * TX + tx;
+        const int j = j0 + ty;
+        const int jw = j < ny ? j : j % ny;
+        const unsigned own_b = SZ * (unsigned)(jw * nx + iw);
+        // Staging loads are buffer loads into LDS: a resource per population array, the
+        // row offset in an SGPR and this lane's column offset in a VGPR that stays live for
+        // the whole kernel (per-load 64-bit VGPR addresses would be recycled by the
+        // compiler right after the load, and a recycled address register of an LDS load
+        // makes it wait for that load).  A halo row (70 columns from i0 - 3) is two 64-lane
+        // loads, columns [i0-3, i0+61) and [i0+3, i0+67) placed at row offsets 0 and 6: the
+        // overlap is written twice with the same values.
+        const unsigned xh = SZ * (unsigned)wrap_index(i0 - 3 + lane, nx);
+        const unsigned xh6 = SZ * (unsigned)wrap_index(i0 + 3 + lane, nx);
+        const unsigned xn = SZ * (unsigned)iw;
+        unsigned yrow[HY];  // byte offset of staged row rr in a plane
+#pragma unroll
+        for (int rr = 0; rr < HY; ++rr) yrow[rr] = SZ * (unsigned)(wrap1(j0 - 3 + rr, ny) * nx);
+        float wx[2][7], wy[2][7];
+#pragma unroll
+        for (int l = 0; l < 7; ++l) {
+            wx[0][l] = a.t.xw0[l];
+            wx[1][l] = a.t.xw2[l];
+            wy[0][l] = a.t.yw0[l];
+            wy[1][l] = a.t.yw2[l];
+        }
 
-    // wave-uniform interpolation weights (uniform x and y mesh)
-    float wx[2][7], wy[2][7];
-#pragma unroll
-    for (int l = 0; l < 7; ++l) {
-        wx[0][l] = a.t.xw0[l];
-        wx[1][l] = a.t.xw2[l];
-        wy[0][l] = a.t.yw0[l];
-        wy[1][l] = a.t.yw2[l];
-    }
-
-    // one staged row: 64 lanes (and the 6-lane tail of a halo row) into LDS
-    auto dma = [&](const raw* src, float* ldst) {
-        if constexpr (sizeof(raw) == 4)
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)ldst, 4, 0, 0);
-        else
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)ldst, 2, 0, 0);
-    };
-    auto dma_row = [&](const raw* rowp, float* ldst, bool halo) {
-        dma(rowp + (halo ? xh : xn), ldst);
-        if (halo && lane < 6) dma(rowp + xt, ldst + 64);
-    };
-    // stage level lev into buffer b; rows are split over the four waves
-    auto fetch = [&](int lev, int b) {
-        const raw* base = a.fo + (long long)lev * plane;
-        float* L = lds + b * STAGE;
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {  // 7-10: HY rows with x halo
-            const raw* qb = base + (long long)kXY[d] * N;
-            for (int rr = wave; rr < HY; rr += 4)
-                dma_row(qb + (long long)wrap1(j0 - 3 + rr, ny) * nx, L + OFF_XY + d * XY_SZ + rr * HX, true);
-        }
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {  // 3, 4, 15-18: HY rows
-            const raw* qb = base + (long long)kY[d] * N;
-            for (int rr = wave; rr < HY; rr += 4)
-                dma_row(qb + (long long)wrap1(j0 - 3 + rr, ny) * nx, L + OFF_Y + d * Y_SZ + rr * TX, false);
-        }
-        {  // 11-14: TY rows with x halo, one direction per wave
-            const int d = wave;
-            const raw* qb = base + (long long)(11 + d) * N;
-#pragma unroll
-            for (int rr = 0; rr < TY; ++rr)
-                dma_row(qb + (long long)wrap1(j0 + rr, ny) * nx, L + OFF_X + d * X_SZ + rr * HX, true);
-        }
-    };
-    // 16-bit storage: the DMA wrote raw codes as dwords; decode to FP32 deviations in place
-    auto decode = [&](int b) {
-        if constexpr (k16) {
+        auto dma = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float* ldst) {
+            auto l = (__attribute__((address_space(3))) void*)ldst;
+            if constexpr (SZ == 4)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(r, l, 4, (int)voff, (int)soff, 0, 0);
+            else
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(r, l, 2, (int)voff, (int)soff, 0, 0);
+        };
+        // this wave's share of level lev: direction 7+w (8 halo rows), 11+w (2 halo rows)
+        // and rows 2w, 2w+1 of each y-halo direction
+        auto fetch = [&](int lev, int b) {
+            const unsigned lv = SZ * (unsigned)lev * (unsigned)plane;
             float* L = lds + b * STAGE;
-            for (int e = t; e < STAGE; e += NTH) {
-                const int q = e < OFF_Y ? kXY[e / XY_SZ] : e < OFF_X ? kY[(e - OFF_Y) / Y_SZ] : kX[(e - OFF_X) / X_SZ];
-                const unsigned short code = (unsigned short)(__float_as_uint(L[e]) & 0xFFFFu);
-                const raw r = __builtin_bit_cast(raw, code);
-                L[e] = CD::ld(&r, 0, a.inv_scale[q]);
+            const auto rd = popr(7 + w);
+#pragma unroll
+            for (int rr = 0; rr < HY; ++rr) {
+                float* dst = L + OFF_XY + w * XY_SZ + rr * HX;
+                dma(rd, xh, lv + yrow[rr], dst);
+                dma(rd, xh6, lv + yrow[rr], dst + 6);
             }
-        }
-    };
-    auto ldq = [&](int q, long long lev, unsigned off) -> float {
-        return CD::ld(a.fo + (long long)q * N + lev * plane, off, a.inv_scale[q]);
-    };
+            const auto rx = popr(11 + w);
+#pragma unroll
+            for (int rr = 0; rr < TY; ++rr) {
+                float* dst = L + OFF_X + w * X_SZ + rr * HX;
+                dma(rx, xh, lv + yrow[3 + rr], dst);
+                dma(rx, xh6, lv + yrow[3 + rr], dst + 6);
+            }
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                const auto ry = popr(qy(d));
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+                    dma(ry, xn, lv + yrow[2 * w + r], L + OFF_Y + d * Y_SZ + (2 * w + r) * TX);
+            }
+        };
+        // 16-bit storage: decode this wave's share to FP32 deviations in place
+        auto decode_region = [&](float* R, int n, float isc) {
+            if constexpr (k16) {
+                for (int e = lane; e < n; e += 64) {
+                    const unsigned short code = (unsigned short)(__float_as_uint(R[e]) & 0xFFFFu);
+                    const raw r = __builtin_bit_cast(raw, code);
+                    R[e] = CD::ld(&r, 0, isc);
+                }
+            }
+        };
+        auto decode = [&](int b) {
+            if constexpr (k16) {
+                float* L = lds + b * STAGE;
+                decode_region(L + OFF_XY + w * XY_SZ, XY_SZ, a.inv_scale[7 + w]);
+                decode_region(L + OFF_X + w * X_SZ, X_SZ, a.inv_scale[11 + w]);
+#pragma unroll
+                for (int d = 0; d < 6; ++d)
+                    decode_region(L + OFF_Y + d * Y_SZ + 2 * w * TX, 2 * TX, a.inv_scale[qy(d)]);
+            }
+        };
 
-    // 7-level ring of this group's 8 directions (kGA or kGB)
-    float g[8][7];
+        auto ring = [&](auto gc) {
+            constexpr int G = decltype(gc)::value;
+            float g[8][7];
 #pragma unroll
-    for (int d = 0; d < 8; ++d)
+            for (int d = 0; d < 8; ++d)
 #pragma unroll
-        for (int m = 0; m < 7; ++m) g[d][m] = 0.f;
+                for (int m = 0; m < 7; ++m) g[d][m] = 0.f;
+            // Vector-memory order per wave: in the step that appends level x, the loads of
+            // directions 5, 6 of level x (group A), then the staging loads of level x+1 (at
+            // the last level, level L1 again into the free buffer); steps that append
+            // nothing issue no vector-memory instruction.  Every appending step thus issues
+            // the same sequence, and the loads of 5, 6 are complete once at most the staging
+            // loads issued after them are outstanding.
+            fetch(L0, 0);
+            int b = 0;  // stage buffer of the next level to append
+            int s = 0;  // step
+            Sched sc{L0 - 1, k0};
+            bool adv;
+            int ho;
+            while (sched_step(sc, k1, nz, adv, ho)) {
+                const int x = sc.top;  // level appended in this step (if adv)
+                if (adv) {
+                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // this wave's share of level x is in LDS
+                    decode(b);
+                }
+                lds_barrier();  // level x staged by all waves; level x-1 and hand-over s-2 no longer read
+                if (adv) {
+                    const float* S = lds + b * STAGE;
+                    float c5 = 0.f, c6 = 0.f;  // directions 5, 6 of level x at the lane's own node
+                    if constexpr (G == 0) {
+                        const unsigned lv = SZ * (unsigned)x * (unsigned)plane;
+                        c5 = IO::ld(popr(5), own_b, lv, a.inv_scale[5]);
+                        c6 = IO::ld(popr(6), own_b, lv, a.inv_scale[6]);
+                        asm volatile("" ::: "memory");  // keep them ahead of the staging loads
+                    }
+                    fetch(min(x + 1, L1), b ^ 1);  // the other buffer holds level x-1
+                    b ^= 1;
 
-    float z5 = 0.f, z6 = 0.f;
-    if (grp == 0) {
-        z5 = ldq(5, L0, own);
-        z6 = ldq(6, L0, own);
-    }
-    fetch(L0, 0);
-    for (int kk = L0; kk <= L1; ++kk) {
-        const int b = (kk - L0) & 1;
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): level kk is in LDS
-        __syncthreads();
-        const float c5 = z5, c6 = z6;
-        const bool more = kk < L1;
-        if (more) {
-            fetch(kk + 1, b ^ 1);  // in flight while level kk is filtered
-            if (grp == 0) {
-                z5 = ldq(5, kk + 1, own);
-                z6 = ldq(6, kk + 1, own);
-            }
-        }
-        if constexpr (k16) {
-            decode(b);
-            __syncthreads();
-        }
-        const float* S = lds + b * STAGE;
-
-        // y pass of 7-10 on the tile's TY rows, all HX columns (shared by the workgroup)
+                    // horizontal filters of this group's directions; shift the ring and append
+                    auto xf = [&](const float* src, const float* wv) {
+                        float sum = src[0] * wv[0];
 #pragma unroll
-        for (int m = 0; m < (4 * YP_SZ + NTH - 1) / NTH; ++m) {
-            const int e = t + m * NTH;
-            if (e < 4 * YP_SZ) {
-                const int d = e / YP_SZ, x = e % YP_SZ;
-                const int r = x / HX, c = x % HX;
-                const float* src = S + OFF_XY + d * XY_SZ + r * HX + c;
-                const bool up = d < 2;  // 7, 8: +y (Y0);  9, 10: -y (Y2)
-                float s = src[0] * (up ? wy[0][0] : wy[1][0]);
+                        for (int l = 1; l < 7; ++l) sum = sum + src[l] * wv[l];
+                        return sum;
+                    };
+                    auto yf = [&](const float* src, const float* wv) {
+                        float sum = src[0] * wv[0];
 #pragma unroll
-                for (int n = 1; n < 7; ++n) s = s + src[n * HX] * (up ? wy[0][n] : wy[1][n]);
-                lds[OFF_YP + e] = s;
-            }
-        }
-        __syncthreads();
-
-        // horizontal filters of this group's directions; shift the ring and append
+                        for (int n = 1; n < 7; ++n) sum = sum + src[n * TX] * wv[n];
+                        return sum;
+                    };
 #pragma unroll
-        for (int d = 0; d < 8; ++d)
+                    for (int d = 0; d < 8; ++d)
 #pragma unroll
-            for (int m = 0; m < 6; ++m) g[d][m] = g[d][m + 1];
-        if (grp == 0) {
-            g[0][6] = c5;
-            g[1][6] = c6;
+                        for (int m = 0; m < 6; ++m) g[d][m] = g[d][m + 1];
+                    if constexpr (G == 0) {
+                        // y pass of 7-10 on this wave's row, all HX columns, into its own LDS
+                        // area (read back by this wave only: no barrier needed)
+                        float* YP = lds + OFF_YP + ty * 4 * HX;
 #pragma unroll
-            for (int d = 0; d < 4; ++d) {  // 7-10: x pass over the y-passed rows
-                const float* w = wx[xsel(kXY[d])];
-                const float* src = lds + OFF_YP + d * YP_SZ + ty * HX + tx;
-                float s = src[0] * w[0];
+                        for (int d = 0; d < 4; ++d) {
+                            const float* wyy = wy[ysel(7 + d)];
+                            for (int c = lane; c < HX; c += 64) {
+                                const float* src = S + OFF_XY + d * XY_SZ + ty * HX + c;
+                                float sum = src[0] * wyy[0];
 #pragma unroll
-                for (int l = 1; l < 7; ++l) s = s + src[l] * w[l];
-                g[2 + d][6] = s;
-            }
+                                for (int n = 1; n < 7; ++n) sum = sum + src[n * HX] * wyy[n];
+                                YP[d * HX + c] = sum;
+                            }
+                        }
+                        g[0][6] = c5;
+                        g[1][6] = c6;
 #pragma unroll
-            for (int d = 0; d < 2; ++d) {  // 11, 12: x filter
-                const float* w = wx[xsel(kX[d])];
-                const float* src = S + OFF_X + d * X_SZ + ty * HX + tx;
-                float s = src[0] * w[0];
+                        for (int d = 0; d < 4; ++d)  // 7-10: x pass over the y-passed row
+                            g[2 + d][6] = xf(YP + d * HX + tx, wx[xsel(7 + d)]);
 #pragma unroll
-                for (int l = 1; l < 7; ++l) s = s + src[l] * w[l];
-                g[6 + d][6] = s;
-            }
-        } else {
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {  // 3, 4: y filter
-                const float* w = wy[ysel(kY[d])];
-                const float* src = S + OFF_Y + d * Y_SZ + ty * TX + tx;
-                float s = src[0] * w[0];
-#pragma unroll
-                for (int n = 1; n < 7; ++n) s = s + src[n * TX] * w[n];
-                g[d][6] = s;
-            }
-#pragma unroll
-            for (int d = 2; d < 4; ++d) {  // 13, 14: x filter
-                const float* w = wx[xsel(kX[d])];
-                const float* src = S + OFF_X + d * X_SZ + ty * HX + tx;
-                float s = src[0] * w[0];
-#pragma unroll
-                for (int l = 1; l < 7; ++l) s = s + src[l] * w[l];
-                g[d][6] = s;
-            }
-#pragma unroll
-            for (int d = 2; d < 6; ++d) {  // 15-18: y filter
-                const float* w = wy[ysel(kY[d])];
-                const float* src = S + OFF_Y + d * Y_SZ + ty * TX + tx;
-                float s = src[0] * w[0];
-#pragma unroll
-                for (int n = 1; n < 7; ++n) s = s + src[n * TX] * w[n];
-                g[2 + d][6] = s;
-            }
-        }
-
-        if (kk - L0 < 6) continue;
-        // targets whose xi stencil starts at base = kk - 6
-        const int base = kk - 6;
-        int tlo, thi;
-        if (base == 0) {
-            tlo = 0; thi = 3;
-        } else if (base == nz - 7) {
-            tlo = nz - 4; thi = nz - 1;
-        } else {
-            tlo = thi = base + 3;
-        }
-        tlo = max(tlo, k0);
-        thi = min(thi, k1);
-        for (int tk = tlo; tk <= thi; ++tk) {
-            const int owner = tk & 1;
-            const float* XI = a.t.xiw + ((long long)tk * ny + jw) * (NXI * 7);
-            float Fm[8];
-#pragma unroll
-            for (int d = 0; d < 8; ++d) {
-                const int q = grp == 0 ? kGA[d] : kGB[d];
-                const float* wz = XI + xifam(q) * 7;
-                float s = g[d][0] * wz[0];
-#pragma unroll
-                for (int m = 1; m < 7; ++m) s = s + g[d][m] * wz[m];
-                Fm[d] = s;
-            }
-            if (grp != owner) {
-#pragma unroll
-                for (int d = 0; d < 8; ++d) lds[OFF_XF + d * (TX * TY) + col] = Fm[d];
-            }
-            __syncthreads();
-            if (grp == owner) {
-                float F[NQ];
-#pragma unroll
-                for (int d = 0; d < 8; ++d) {
-                    const float o = lds[OFF_XF + d * (TX * TY) + col];
-                    if (grp == 0) {
-                        F[kGA[d]] = Fm[d];
-                        F[kGB[d]] = o;
+                        for (int d = 0; d < 2; ++d)  // 11, 12: x filter
+                            g[6 + d][6] = xf(S + OFF_X + d * X_SZ + ty * HX + tx, wx[xsel(11 + d)]);
                     } else {
-                        F[kGB[d]] = Fm[d];
-                        F[kGA[d]] = o;
+#pragma unroll
+                        for (int d = 0; d < 2; ++d)  // 3, 4: y filter
+                            g[d][6] = yf(S + OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(3 + d)]);
+#pragma unroll
+                        for (int d = 2; d < 4; ++d)  // 13, 14: x filter
+                            g[d][6] = xf(S + OFF_X + d * X_SZ + ty * HX + tx, wx[xsel(11 + d)]);
+#pragma unroll
+                        for (int d = 2; d < 6; ++d)  // 15-18: y filter
+                            g[2 + d][6] = yf(S + OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]);
                     }
                 }
-                const unsigned row = (unsigned)(jw * nx);
-                F[0] = ldq(0, tk, own);
-                {
-                    float s1 = 0.f, s2 = 0.f;
+                if (ho >= 0) {
+                    // xi weights of the target (wave-uniform): read through the constant
+                    // address space, so they arrive by scalar loads and the vector-memory
+                    // count above holds only the staging loads
+                    const __attribute__((address_space(4))) float* XI =
+                        (const __attribute__((address_space(4))) float*)a.t.xiw + ((long long)ho * ny + jw) * (NXI * 7);
+                    float* HO = lds + OFF_HO + (s & 1) * HO_SZ;
 #pragma unroll
-                    for (int l = 0; l < 7; ++l) {
-                        const unsigned o = row + (unsigned)wrap_index(iw - 3 + l, nx);
-                        s1 = (l == 0) ? ldq(1, tk, o) * wx[0][0] : s1 + ldq(1, tk, o) * wx[0][l];
-                        s2 = (l == 0) ? ldq(2, tk, o) * wx[1][0] : s2 + ldq(2, tk, o) * wx[1][l];
+                    for (int d = 0; d < 8; ++d) {
+                        const int q = gq(G, d);
+                        const __attribute__((address_space(4))) float* wz = XI + xifam(q) * 7;
+                        float sum = g[d][0] * wz[0];
+#pragma unroll
+                        for (int m = 1; m < 7; ++m) sum = sum + g[d][m] * wz[m];
+                        HO[(q - 3) * (TX * TY) + col] = sum;
                     }
-                    F[1] = s1;
-                    F[2] = s2;
                 }
-                const long long node = (long long)tk * plane + own;
-                if (tk <= 1 || tk == nz - 1) {
-                    int xo[7], yo[7];
-#pragma unroll
-                    for (int l = 0; l < 7; ++l) {
-                        xo[l] = wrap_index(iw - 3 + l, nx);
-                        yo[l] = wrap_index(jw - 3 + l, ny) * nx;
-                    }
-                    apply_walls<ST>(F, a, node, iw, jw, tk, xo, yo, plane);
-                }
-                float rho, ux, uy, uz, tau_eff;
-                collide_pairs<true>(F, (float)*a.rho_modify, (float)a.dt, (float)(*a.force),
-                                    a.les ? a.t.les_delta[(long long)tk * ny + jw] : 0.f, a.les, (float)a.tau,
-                                    (float)a.cs, rho, ux, uy, uz, tau_eff);
-                if (active) {
-                    raw* fnb = a.fn + (long long)tk * plane;
-#pragma unroll
-                    for (int q = 0; q < NQ; ++q) CD::st(fnb + (long long)q * N, own, F[q], a.scale[q]);
-                    (a.rho + (long long)tk * plane)[own] = rho;
-                    (a.u + (long long)tk * plane)[own] = ux;
-                    (a.v + (long long)tk * plane)[own] = uy;
-                    (a.w + (long long)tk * plane)[own] = uz;
-                }
+                ++s;
             }
-            if (tk < thi) __syncthreads();  // the hand-over buffer is reused by the next target
+            lds_barrier();  // the final step, in which the collision waves take the last target
+            // no staging load may still write LDS when the workgroup retires
+            __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+        };
+        if (w < TY)
+            ring(std::integral_constant<int, 0>{});
+        else
+            ring(std::integral_constant<int, 1>{});
+    } else {
+        // ------------------------------------------------------------- collision waves
+        // In step s they collide the target handed over in step s-1, while the ring waves
+        // filter and hand over the next one.
+        const int ty = wave - NRW;
+        const int col = ty * TX + tx;
+        const int j = j0 + ty;
+        const int jw = j < ny ? j : j % ny;
+        const unsigned own = (unsigned)(jw * nx + iw);
+        const unsigned own_b = SZ * own;
+        const unsigned own4 = 4u * own;
+        const unsigned row = (unsigned)(jw * nx);
+        unsigned xo12[7];  // byte offsets of the 7 x taps in the plane
+#pragma unroll
+        for (int l = 0; l < 7; ++l) xo12[l] = SZ * (row + (unsigned)wrap_index(iw - 3 + l, nx));
+        float wx0[7], wx2[7];
+#pragma unroll
+        for (int l = 0; l < 7; ++l) {
+            wx0[l] = a.t.xw0[l];
+            wx2[l] = a.t.xw2[l];
+        }
+        const float isc0 = a.inv_scale[0], isc1 = a.inv_scale[1], isc2 = a.inv_scale[2];
+        // Scalars and tables that no kernel writes during the step are read through the
+        // constant address space (scalar loads): a vector load here would be waited for
+        // together with the next target's loads issued just before it.
+        using cdouble = const __attribute__((address_space(4))) double;
+        const float rho_mod = (float)*(cdouble*)a.rho_modify;
+        const float force = (float)*(cdouble*)a.force;
+        const int fbytes = (int)(4u * (unsigned)N);
+        // directions 0, 1, 2 of a target level (no z dependence): own node and 7 x taps
+        float p0 = 0.f, p1[7], p2[7];
+        auto load012 = [&](int tk) {
+            const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
+            p0 = IO::ld(popr(0), own_b, lv, isc0);
+            const auto r1 = popr(1), r2 = popr(2);
+#pragma unroll
+            for (int l = 0; l < 7; ++l) {
+                p1[l] = IO::ld(r1, xo12[l], lv, isc1);
+                p2[l] = IO::ld(r2, xo12[l], lv, isc2);
+            }
+        };
+        // the taps of direction 1, 2 and the node of direction 0, reduced to F0, F1, F2
+        auto reduce012 = [&](float (&F)[NQ]) {
+            F[0] = p0;
+            float s1 = p1[0] * wx0[0], s2 = p2[0] * wx2[0];
+#pragma unroll
+            for (int l = 1; l < 7; ++l) {
+                s1 = s1 + p1[l] * wx0[l];
+                s2 = s2 + p2[l] * wx2[l];
+            }
+            F[1] = s1;
+            F[2] = s2;
+        };
+        // walls, collision and stores of target level tk.  Lanes outside the grid (partial
+        // tiles) hold the wrapped column or row, whose values they compute bit for bit like
+        // its owner, so their stores duplicate the owner's: the stores need no lane mask,
+        // and every step issues the same vector-memory sequence.
+        auto collide_store = [&](float (&F)[NQ], int tk) {
+            const long long node = (long long)tk * plane + own;
+            if (tk <= 1 || tk == nz - 1) {
+                int xo[7], yo[7];
+#pragma unroll
+                for (int l = 0; l < 7; ++l) {
+                    xo[l] = wrap_index(iw - 3 + l, nx);
+                    yo[l] = wrap_index(jw - 3 + l, ny) * nx;
+                }
+                apply_walls<ST>(F, a, node, iw, jw, tk, xo, yo, plane);
+            }
+            const float delta = a.les ? ((const __attribute__((address_space(4))) float*)a.t.les_delta)[(long long)tk * ny + jw]
+                                      : 0.f;
+            float rho, ux, uy, uz, tau_eff;
+            collide_pairs<true>(F, rho_mod, (float)a.dt, force, delta, a.les, (float)a.tau, (float)a.cs, rho, ux, uy, uz,
+                                tau_eff);
+            const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) IO::st(rsrc_of(a.fn + (long long)q * N, nbytes), own_b, lv, F[q], a.scale[q]);
+            const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rsrc_of(a.rho, fbytes), (int)own4, (int)lf, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), rsrc_of(a.u, fbytes), (int)own4, (int)lf, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rsrc_of(a.v, fbytes), (int)own4, (int)lf, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rsrc_of(a.w, fbytes), (int)own4, (int)lf, 0);
+        };
+        Sched sc{L0 - 1, k0};
+        bool adv;
+        int ho;
+        int s = 0;
+        // warm-up: the steps up to the first hand-over
+        for (;;) {
+            sched_step(sc, k1, nz, adv, ho);
+            lds_barrier();
+            ++s;
+            if (ho >= 0) break;
+        }
+        int pend = ho;  // target handed over in step s-1
+        load012(pend);
+        // one target per step; the loads of the next target go out before the stores of
+        // this one, so that waiting for them does not wait for the stores
+        for (;;) {
+            const bool more = sched_step(sc, k1, nz, adv, ho);
+            lds_barrier();  // hand-over of pend (step s-1) written
+            float F[NQ];
+            const float* HO = lds + OFF_HO + ((s - 1) & 1) * HO_SZ;
+#pragma unroll
+            for (int q = 3; q < NQ; ++q) F[q] = HO[(q - 3) * (TX * TY) + col];
+            reduce012(F);
+            if (!more) {  // final step: the ring has appended and handed over everything
+                collide_store(F, pend);
+                break;
+            }
+            load012(ho);
+            asm volatile("" ::: "memory");  // keep the loads ahead of the stores
+            collide_store(F, pend);
+            pend = ho;
+            ++s;
         }
     }
 }
 
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
 {
-    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8;
+    // buffer resources address one population or field array with 32-bit byte offsets
+    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && g.n * 4 < (1LL << 31);
 }
 
 template <Store ST>

@@ -204,7 +204,8 @@ def test_kernel_is_native(pkg):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
-@pytest.mark.parametrize("shape", [dict(nx=64, ny=32, nz=32), dict(nx=48, ny=20, nz=24), dict(nx=16, ny=32, nz=64)])
+@pytest.mark.parametrize("shape", [dict(nx=64, ny=32, nz=32), dict(nx=48, ny=20, nz=24), dict(nx=16, ny=32, nz=64),
+                                   dict(nx=64, ny=32, nz=48, cs_smag=0.1)])
 def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
     """Column-marching kernel (z chunks, wrapped partial tiles) against the FP64 oracle."""
     o, s = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED, **shape)

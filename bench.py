@@ -82,7 +82,7 @@ def main() -> int:
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
     ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled"])
     ap.add_argument("--physics", default="stable-les", choices=sorted(PHYSICS))
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

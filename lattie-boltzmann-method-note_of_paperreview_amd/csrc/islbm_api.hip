@@ -81,7 +81,7 @@ size_t table_bytes(const islbm_params& p)
     b += NBFL * 2 * ny * (sizeof(int) + cb);         // BFL flags, q
     b += (size_t)NBW * 2 * ny * 7 * cb;              // BFL weights
     b += nz * ny * cb;                               // LES filter width
-    b += nz * nx * 8;                                // flux weights
+    b += nz * nx * 8 + nz * 8;                       // flux weights, per-level flux partials
     b += 8 * 8;                                      // scalars
     return b;
 }
@@ -163,7 +163,8 @@ int upload_tables(islbm_solver* s)
         code[t] = !g.bflag[t] ? 0 : (g.bq[t] > 0.5 ? 1 : (g.bq[t] < 0.5 ? 2 : 3));
     HIPCHK(hipMalloc((void**)&s->bflag, code.size() * sizeof(int)));
     HIPCHK(hipMemcpy(s->bflag, code.data(), code.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc((void**)&s->fluxw, g.fluxw.size() * sizeof(double)));
+    // flux weights [nz][nx], followed by nz per-level partial sums of the flux probe
+    HIPCHK(hipMalloc((void**)&s->fluxw, (g.fluxw.size() + g.nz) * sizeof(double)));
     HIPCHK(hipMemcpy(s->fluxw, g.fluxw.data(), g.fluxw.size() * sizeof(double), hipMemcpyHostToDevice));
 
     // per-node xi weights: FP64 on the device, then stored in compute precision
@@ -318,8 +319,8 @@ int enqueue_force_update(islbm_solver* s)
 // one Launch_CollisionStreaming; ev (optional) brackets the stream-collide launch
 int enqueue_step(islbm_solver* s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    HIPCHK(launch_flux_probe(s->stream, s->mac[2], s->fp64, s->fluxw, s->scal + 2, s->p.nx, s->p.ny, s->p.nz,
-                             s->g.flux_plane));
+    HIPCHK(launch_flux_probe(s->stream, s->mac[2], s->fp64, s->fluxw, s->fluxw + (size_t)s->p.nx * s->p.nz, s->scal + 2,
+                             s->p.nx, s->p.ny, s->p.nz, s->g.flux_plane));
     if (ev0) HIPCHK(hipEventRecord(ev0, s->stream));
     StepLaunch L = make_launch(s);
     HIPCHK(launch_stream_collide(s->stream, L));

@@ -80,7 +80,7 @@ hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L);
 hipError_t launch_stream_copy(hipStream_t st, const void* src, void* dst, long long bytes);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
 
-hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw,
+hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* part,
                              double* acc, int nx, int ny, int nz, int jplane);
 hipError_t launch_force_update(hipStream_t st, double* force, double* acc, double inv_area,
                                int ndtfrc, double beta, double uref);

@@ -8,7 +8,7 @@
 //                       with body force (:597-679, MRT_Process.h:4-179), optional
 //                       Smagorinsky relaxation.  x and y periodicity by index wrap, which
 //                       replaces periodicSW (:732-777) and the y halo exchange (:980-994).
-//   k_flux_probe        FP64 bulk-flux sum over the probe plane (AccumulateUbulk, :826-840).
+//   k_flux_partial/final FP64 bulk-flux sum over the probe plane (AccumulateUbulk, :826-840).
 //   k_force_update      forcing controller (Launch_ModifyForcingTerm, :1003-1042).
 //
 // Population storage is [q][k][j][i].  FP64 stores f itself; FP32 stores f - w_q; the
@@ -130,16 +130,31 @@ __global__ __launch_bounds__(256) void k_collide_pernode(StepArgs<ST> a)
 // Bulk flux and forcing controller
 // ------------------------------------------------------------------------------------
 
-template <typename M>
-__global__ __launch_bounds__(256) void k_flux_probe(const M* __restrict__ v, const double* __restrict__ fw,
-                                                    double* acc, int nx, int ny, int nz, int jp)
+// Bulk flux through the probe plane in two deterministic passes: one block per z level
+// sums its row in a fixed order, then one block sums the levels in index order.
+template <class M>
+__global__ __launch_bounds__(256) void k_flux_partial(const M* __restrict__ v, const double* __restrict__ fw,
+                                                      double* __restrict__ part, int nx, int ny, int jp)
+{
+    __shared__ double red[256];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+        s += (double)v[((long long)k * ny + jp) * nx + i] * fw[(long long)k * nx + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[k] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_flux_final(const double* __restrict__ part, int nz, double* acc)
 {
     __shared__ double red[256];
     double s = 0.0;
-    for (int t = threadIdx.x; t < nx * nz; t += blockDim.x) {
-        const int i = t % nx, k = t / nx;
-        s += (double)v[((long long)k * ny + jp) * nx + i] * fw[t];
-    }
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) s += part[k];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -149,15 +164,16 @@ __global__ __launch_bounds__(256) void k_flux_probe(const M* __restrict__ v, con
     if (threadIdx.x == 0) acc[0] += red[0];
 }
 
-hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* acc, int nx,
-                             int ny, int nz, int jplane)
+hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* part, double* acc,
+                             int nx, int ny, int nz, int jplane)
 {
     if (fp64)
-        hipLaunchKernelGGL(k_flux_probe<double>, dim3(1), dim3(256), 0, st, (const double*)v, fluxw, acc, nx, ny, nz,
+        hipLaunchKernelGGL(k_flux_partial<double>, dim3(nz), dim3(256), 0, st, (const double*)v, fluxw, part, nx, ny,
                            jplane);
     else
-        hipLaunchKernelGGL(k_flux_probe<float>, dim3(1), dim3(256), 0, st, (const float*)v, fluxw, acc, nx, ny, nz,
+        hipLaunchKernelGGL(k_flux_partial<float>, dim3(nz), dim3(256), 0, st, (const float*)v, fluxw, part, nx, ny,
                            jplane);
+    hipLaunchKernelGGL(k_flux_final, dim3(1), dim3(256), 0, st, (const double*)part, nz, acc);
     return hipGetLastError();
 }
 

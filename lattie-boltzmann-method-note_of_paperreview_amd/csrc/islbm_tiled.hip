@@ -53,7 +53,8 @@ constexpr int STAGE = OFF_X + 4 * X_SZ;  // 5872 staged values per level
 constexpr int OFF_YP = 2 * STAGE;        // group A: y pass of 7-10 for its row, 4 x HX per wave
 constexpr int HO_SZ = 16 * TX * TY;      // hand-over: directions 3-18 x 128 columns
 constexpr int OFF_HO = OFF_YP + TY * 4 * HX;
-constexpr int LDS_FLOATS = OFF_HO + 2 * HO_SZ;  // 16400 floats = 64 KB: two workgroups per CU
+constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps x TY rows x 64
+constexpr int LDS_FLOATS = OFF_XIB + 2 * TY * 64;  // 16656 floats = 65 KB: two workgroups per CU
 static_assert(TY == 2, "the fetch split below assumes two rows per tile");
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
 
@@ -188,6 +189,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
     const long long plane = (long long)nx * ny;
     const long long N = a.n;
     const int nbytes = (int)(SZ * (unsigned)N);  // one population array (tiled_supported: < 2^31)
+    const int xi_bytes = (int)(4u * (unsigned)nz * (unsigned)ny * (NXI * 7));
     const int lane = threadIdx.x % 64;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int k0 = chunk * tg.kc;
@@ -303,16 +305,34 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
             Sched sc{L0 - 1, k0};
             bool adv;
             int ho;
+            // xi weights of each step's target are staged into LDS one step ahead by the
+            // group A waves (one 64-lane load of the 56 weights of its row; the last 8 lanes
+            // read past the row and are not used)
+            const auto rxi = rsrc_of(a.t.xiw, xi_bytes);
+            auto fetch_xi = [&](int tk, int par) {
+                if constexpr (G == 0)  // an FP32 table whatever the population storage
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + ty) * 64), 4,
+                        (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
+            };
+            {
+                Sched nx0 = sc;
+                bool a0;
+                int h0;
+                sched_step(nx0, k1, nz, a0, h0);
+                fetch_xi(h0 >= 0 ? h0 : k0, 0);
+            }
             while (sched_step(sc, k1, nz, adv, ho)) {
                 const int x = sc.top;  // level appended in this step (if adv)
-                if (adv) {
-                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // this wave's share of level x is in LDS
-                    decode(b);
+                if (adv || G == 0) {
+                    // this wave's share of level x (and, group A, the xi weights of this step's target) is in LDS
+                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+                    if (adv) decode(b);
                 }
                 lds_barrier();  // level x staged by all waves; level x-1 and hand-over s-2 no longer read
+                const float* S = lds + b * STAGE;  // level x (steps that append)
+                float c5 = 0.f, c6 = 0.f;          // directions 5, 6 of level x at the lane's own node
                 if (adv) {
-                    const float* S = lds + b * STAGE;
-                    float c5 = 0.f, c6 = 0.f;  // directions 5, 6 of level x at the lane's own node
                     if constexpr (G == 0) {
                         const unsigned lv = SZ * (unsigned)x * (unsigned)plane;
                         c5 = IO::ld(popr(5), own_b, lv, a.inv_scale[5]);
@@ -321,7 +341,15 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                     }
                     fetch(min(x + 1, L1), b ^ 1);  // the other buffer holds level x-1
                     b ^= 1;
-
+                }
+                {
+                    Sched nx1 = sc;
+                    bool a1;
+                    int h1;
+                    sched_step(nx1, k1, nz, a1, h1);
+                    fetch_xi(h1 >= 0 ? h1 : k0, (s + 1) & 1);  // read by step s+1 after its barrier
+                }
+                if (adv) {
                     // horizontal filters of this group's directions; shift the ring and append
                     auto xf = [&](const float* src, const float* wv) {
                         float sum = src[0] * wv[0];
@@ -375,16 +403,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                     }
                 }
                 if (ho >= 0) {
-                    // xi weights of the target (wave-uniform): read through the constant
-                    // address space, so they arrive by scalar loads and the vector-memory
-                    // count above holds only the staging loads
-                    const __attribute__((address_space(4))) float* XI =
-                        (const __attribute__((address_space(4))) float*)a.t.xiw + ((long long)ho * ny + jw) * (NXI * 7);
+                    // xi weights of the target (wave-uniform LDS reads, staged in step s-1)
+                    const float* XI = lds + OFF_XIB + ((s & 1) * TY + ty) * 64;
                     float* HO = lds + OFF_HO + (s & 1) * HO_SZ;
 #pragma unroll
                     for (int d = 0; d < 8; ++d) {
                         const int q = gq(G, d);
-                        const __attribute__((address_space(4))) float* wz = XI + xifam(q) * 7;
+                        const float* wz = XI + xifam(q) * 7;
                         float sum = g[d][0] * wz[0];
 #pragma unroll
                         for (int m = 1; m < 7; ++m) sum = sum + g[d][m] * wz[m];
@@ -522,7 +547,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
 {
     // buffer resources address one population or field array with 32-bit byte offsets
-    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && g.n * 4 < (1LL << 31);
+    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && g.n * 4 < (1LL << 31) &&
+           (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 31);
 }
 
 template <Store ST>

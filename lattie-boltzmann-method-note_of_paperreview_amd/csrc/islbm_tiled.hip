@@ -54,7 +54,12 @@ constexpr int OFF_YP = 2 * STAGE;        // group A: y pass of 7-10 for its row,
 constexpr int HO_SZ = 16 * TX * TY;      // hand-over: directions 3-18 x 128 columns
 constexpr int OFF_HO = OFF_YP + TY * 4 * HX;
 constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps x TY rows x 64
-constexpr int LDS_FLOATS = OFF_XIB + 2 * TY * 64;  // 16656 floats = 65 KB: two workgroups per CU
+// directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
+constexpr int P_ROW = TX + 2 * HX;
+constexpr int OFF_P012 = OFF_XIB + 2 * TY * 64;
+constexpr int LDS_FLOATS = OFF_P012 + 2 * TY * P_ROW;  // 17472 floats = 68 KB: two workgroups per CU
+// cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
+constexpr int kStoreAux = 18;
 static_assert(TY == 2, "the fetch split below assumes two rows per tile");
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
 
@@ -122,9 +127,9 @@ struct BufIO {
         raw v;
         CD::st(&v, 0, x, sc);
         if constexpr (sizeof(raw) == 4)
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, kStoreAux);
         else
-            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)voff, (int)soff, 0);
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)voff, (int)soff, kStoreAux);
     }
 };
 
@@ -315,6 +320,23 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                         rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + ty) * 64), 4,
                         (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
             };
+            // rows of directions 0, 1, 2 of target tk for the collision waves: waves 0, 1 stage
+            // 0 and 1 of row w, waves 2, 3 stage 2 of row w-2 (no z dependence)
+            auto fetch012 = [&](int tk, int par) {
+                const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
+                const int r = w % TY;
+                float* P = lds + OFF_P012 + (par * TY + r) * P_ROW;
+                if (w < TY) {
+                    dma(popr(0), xn, lv + yrow[3 + r], P);
+                    const auto r1 = popr(1);
+                    dma(r1, xh, lv + yrow[3 + r], P + TX);
+                    dma(r1, xh6, lv + yrow[3 + r], P + TX + 6);
+                } else {
+                    const auto r2 = popr(2);
+                    dma(r2, xh, lv + yrow[3 + r], P + TX + HX);
+                    dma(r2, xh6, lv + yrow[3 + r], P + TX + HX + 6);
+                }
+            };
             {
                 Sched nx0 = sc;
                 bool a0;
@@ -324,11 +346,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
             while (sched_step(sc, k1, nz, adv, ho)) {
                 const int x = sc.top;  // level appended in this step (if adv)
-                if (adv || G == 0) {
-                    // this wave's share of level x (and, group A, the xi weights of this step's target) is in LDS
-                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-                    if (adv) decode(b);
-                }
+                // this wave's share of level x and of the rows of directions 0-2 and xi weights
+                // staged for this step are in LDS
+                __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+                if (adv) decode(b);
                 lds_barrier();  // level x staged by all waves; level x-1 and hand-over s-2 no longer read
                 const float* S = lds + b * STAGE;  // level x (steps that append)
                 float c5 = 0.f, c6 = 0.f;          // directions 5, 6 of level x at the lane's own node
@@ -349,6 +370,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                     sched_step(nx1, k1, nz, a1, h1);
                     fetch_xi(h1 >= 0 ? h1 : k0, (s + 1) & 1);  // read by step s+1 after its barrier
                 }
+                if (ho >= 0) fetch012(ho, s & 1);  // read by the collision waves in step s+1
                 if (adv) {
                     // horizontal filters of this group's directions; shift the ring and append
                     auto xf = [&](const float* src, const float* wv) {
@@ -418,9 +440,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
                 ++s;
             }
-            lds_barrier();  // the final step, in which the collision waves take the last target
-            // no staging load may still write LDS when the workgroup retires
+            // the final step, in which the collision waves take the last target: its rows of
+            // 0-2 must be in LDS before the barrier, and no staging load may still write LDS
+            // when the workgroup retires
             __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+            lds_barrier();
         };
         if (w < TY)
             ring(std::integral_constant<int, 0>{});
@@ -438,9 +462,6 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         const unsigned own_b = SZ * own;
         const unsigned own4 = 4u * own;
         const unsigned row = (unsigned)(jw * nx);
-        unsigned xo12[7];  // byte offsets of the 7 x taps in the plane
-#pragma unroll
-        for (int l = 0; l < 7; ++l) xo12[l] = SZ * (row + (unsigned)wrap_index(iw - 3 + l, nx));
         float wx0[7], wx2[7];
 #pragma unroll
         for (int l = 0; l < 7; ++l) {
@@ -455,26 +476,25 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         const float rho_mod = (float)*(cdouble*)a.rho_modify;
         const float force = (float)*(cdouble*)a.force;
         const int fbytes = (int)(4u * (unsigned)N);
-        // directions 0, 1, 2 of a target level (no z dependence): own node and 7 x taps
-        float p0 = 0.f, p1[7], p2[7];
-        auto load012 = [&](int tk) {
-            const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
-            p0 = IO::ld(popr(0), own_b, lv, isc0);
-            const auto r1 = popr(1), r2 = popr(2);
-#pragma unroll
-            for (int l = 0; l < 7; ++l) {
-                p1[l] = IO::ld(r1, xo12[l], lv, isc1);
-                p2[l] = IO::ld(r2, xo12[l], lv, isc2);
+        // directions 0, 1, 2 of the target (no z dependence), staged in LDS by the ring
+        // waves: the node's own 0 and the 7 x taps of 1 and 2, reduced to F0, F1, F2
+        auto dec = [&](float v, float isc) -> float {
+            if constexpr (k16) {
+                const raw r = __builtin_bit_cast(raw, (unsigned short)(__float_as_uint(v) & 0xFFFFu));
+                return CD::ld(&r, 0, isc);
+            } else {
+                return v;
             }
         };
-        // the taps of direction 1, 2 and the node of direction 0, reduced to F0, F1, F2
-        auto reduce012 = [&](float (&F)[NQ]) {
-            F[0] = p0;
-            float s1 = p1[0] * wx0[0], s2 = p2[0] * wx2[0];
+        auto reduce012 = [&](float (&F)[NQ], const float* P) {
+            F[0] = dec(P[tx], isc0);
+            const float* P1 = P + TX + tx;
+            const float* P2 = P + TX + HX + tx;
+            float s1 = dec(P1[0], isc1) * wx0[0], s2 = dec(P2[0], isc2) * wx2[0];
 #pragma unroll
             for (int l = 1; l < 7; ++l) {
-                s1 = s1 + p1[l] * wx0[l];
-                s2 = s2 + p2[l] * wx2[l];
+                s1 = s1 + dec(P1[l], isc1) * wx0[l];
+                s2 = s2 + dec(P2[l], isc2) * wx2[l];
             }
             F[1] = s1;
             F[2] = s2;
@@ -503,10 +523,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
             for (int q = 0; q < NQ; ++q) IO::st(rsrc_of(a.fn + (long long)q * N, nbytes), own_b, lv, F[q], a.scale[q]);
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rsrc_of(a.rho, fbytes), (int)own4, (int)lf, 0);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), rsrc_of(a.u, fbytes), (int)own4, (int)lf, 0);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rsrc_of(a.v, fbytes), (int)own4, (int)lf, 0);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rsrc_of(a.w, fbytes), (int)own4, (int)lf, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rsrc_of(a.rho, fbytes), (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), rsrc_of(a.u, fbytes), (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rsrc_of(a.v, fbytes), (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rsrc_of(a.w, fbytes), (int)own4, (int)lf, kStoreAux);
         };
         Sched sc{L0 - 1, k0};
         bool adv;
@@ -520,24 +540,19 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (ho >= 0) break;
         }
         int pend = ho;  // target handed over in step s-1
-        load012(pend);
-        // one target per step; the loads of the next target go out before the stores of
-        // this one, so that waiting for them does not wait for the stores
+        // One target per step.  The collision waves issue no global loads: a wait for a load
+        // would also wait for every store issued before it (one in-order counter), so their
+        // inputs all come through LDS.
         for (;;) {
             const bool more = sched_step(sc, k1, nz, adv, ho);
-            lds_barrier();  // hand-over of pend (step s-1) written
+            lds_barrier();  // hand-over of pend (step s-1) and its rows of 0-2 in LDS
             float F[NQ];
             const float* HO = lds + OFF_HO + ((s - 1) & 1) * HO_SZ;
 #pragma unroll
             for (int q = 3; q < NQ; ++q) F[q] = HO[(q - 3) * (TX * TY) + col];
-            reduce012(F);
-            if (!more) {  // final step: the ring has appended and handed over everything
-                collide_store(F, pend);
-                break;
-            }
-            load012(ho);
-            asm volatile("" ::: "memory");  // keep the loads ahead of the stores
+            reduce012(F, lds + OFF_P012 + (((s - 1) & 1) * TY + ty) * P_ROW);
             collide_store(F, pend);
+            if (!more) break;  // final step: the ring has appended and handed over everything
             pend = ho;
             ++s;
         }

@@ -227,3 +227,35 @@ def test_tiled_equals_pernode_fp32(pkg, po):
         x.step(20)
     fa, fb = a.f(), b.f()
     assert np.max(np.abs(fa - fb)) < 5e-6
+
+
+# --- production shape: reduced precision against FP64 on the same seeded field ---------
+PROD = dict(nx=128, ny=256, nz=128, tau=1.0, re=50.0, u_ref=0.0, cs_smag=0.1, mesh_mode=0)
+
+
+@pytest.fixture(scope="module")
+def prod_fp64(pkg):
+    """FP64 per-node kernel (equal to the oracle to 1e-10 on the parity grids) after 200
+    steps at 128 x 256 x 128 from u = (0, U_ref, 0) + a seeded 1e-3 perturbation."""
+    s = pkg.Solver(pkg.default_params(**PROD, precision="fp64", kernel=pkg.KERNEL_PERNODE))
+    s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    s.step(200)
+    m = s.macros()
+    s.close()
+    return m
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("fp16", 6e-3), ("bf16", 2e-2)])
+def test_velocity_error_at_production_shape(pkg, prod_fp64, prec, tol):
+    """Relative L2 velocity error of the tiled kernel against FP64 at 128 x 256 x 128 after
+    200 steps: FP32 within 1e-5; 16-bit storage within the stated bounds."""
+    s = pkg.Solver(pkg.default_params(**PROD, precision=prec))
+    assert s.kernel_in_use() == pkg.KERNEL_TILED
+    s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    s.step(200)
+    r, u, v, w = s.macros()
+    ro, uo, vo, wo = prod_fp64
+    e_u = float(np.sqrt(np.sum((u - uo) ** 2 + (v - vo) ** 2 + (w - wo) ** 2) / np.sum(uo ** 2 + vo ** 2 + wo ** 2)))
+    e_rho = float(np.linalg.norm(r - ro) / np.linalg.norm(ro))
+    print(f"{prec} 128x256x128 200 steps: rel L2 u {e_u:.3e} rho {e_rho:.3e}", flush=True)
+    assert e_u <= tol and e_rho <= tol, (e_u, e_rho)

@@ -100,10 +100,12 @@ TiledGeom tiled_geom(int nx, int ny, int nz)
 
 __device__ inline int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// Buffer resource over one population (or field) array: 32-bit byte offsets from its start.
-__device__ inline __amdgpu_buffer_rsrc_t rsrc_of(const void* base, int nbytes)
+// Buffer resource over a buffer of populations, a field or a table: 32-bit byte offsets
+// from its start.  One resource serves all 19 arrays of a population buffer (the direction
+// goes into the scalar offset), so the kernel holds a handful of resources in SGPRs.
+__device__ inline __amdgpu_buffer_rsrc_t rsrc_of(const void* base, unsigned nbytes)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nbytes, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes, 0x00020000);
 }
 
 // One population value through a buffer resource, decoded to / encoded from the FP32
@@ -193,8 +195,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long long plane = (long long)nx * ny;
     const long long N = a.n;
-    const int nbytes = (int)(SZ * (unsigned)N);  // one population array (tiled_supported: < 2^31)
-    const int xi_bytes = (int)(4u * (unsigned)nz * (unsigned)ny * (NXI * 7));
+    const unsigned nbq = SZ * (unsigned)N;  // bytes of one population array; 19 of them < 2^32 (tiled_supported)
+    const unsigned xi_bytes = 4u * (unsigned)nz * (unsigned)ny * (NXI * 7);
     const int lane = threadIdx.x % 64;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int k0 = chunk * tg.kc;
@@ -204,7 +206,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int tx = lane;
     const int i = i0 + tx;
     const int iw = i < nx ? i : i % nx;
-    auto popr = [&](int q) { return rsrc_of(a.fo + (long long)q * N, nbytes); };
+    const auto rfo = rsrc_of(a.fo, NQ * nbq);
+    auto qoff = [&](int q) { return (unsigned)q * nbq; };
 
     if (wave < NRW) {
         // ------------------------------------------------------------------ ring waves
@@ -248,26 +251,26 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         auto fetch = [&](int lev, int b) {
             const unsigned lv = SZ * (unsigned)lev * (unsigned)plane;
             float* L = lds + b * STAGE;
-            const auto rd = popr(7 + w);
+            const unsigned od = qoff(7 + w) + lv;
 #pragma unroll
             for (int rr = 0; rr < HY; ++rr) {
                 float* dst = L + OFF_XY + w * XY_SZ + rr * HX;
-                dma(rd, xh, lv + yrow[rr], dst);
-                dma(rd, xh6, lv + yrow[rr], dst + 6);
+                dma(rfo, xh, od + yrow[rr], dst);
+                dma(rfo, xh6, od + yrow[rr], dst + 6);
             }
-            const auto rx = popr(11 + w);
+            const unsigned ox = qoff(11 + w) + lv;
 #pragma unroll
             for (int rr = 0; rr < TY; ++rr) {
                 float* dst = L + OFF_X + w * X_SZ + rr * HX;
-                dma(rx, xh, lv + yrow[3 + rr], dst);
-                dma(rx, xh6, lv + yrow[3 + rr], dst + 6);
+                dma(rfo, xh, ox + yrow[3 + rr], dst);
+                dma(rfo, xh6, ox + yrow[3 + rr], dst + 6);
             }
 #pragma unroll
             for (int d = 0; d < 6; ++d) {
-                const auto ry = popr(qy(d));
+                const unsigned oy = qoff(qy(d)) + lv;
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
-                    dma(ry, xn, lv + yrow[2 * w + r], L + OFF_Y + d * Y_SZ + (2 * w + r) * TX);
+                    dma(rfo, xn, oy + yrow[2 * w + r], L + OFF_Y + d * Y_SZ + (2 * w + r) * TX);
             }
         };
         // 16-bit storage: decode this wave's share to FP32 deviations in place
@@ -326,15 +329,14 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
                 const int r = w % TY;
                 float* P = lds + OFF_P012 + (par * TY + r) * P_ROW;
+                const unsigned o = lv + yrow[3 + r];
                 if (w < TY) {
-                    dma(popr(0), xn, lv + yrow[3 + r], P);
-                    const auto r1 = popr(1);
-                    dma(r1, xh, lv + yrow[3 + r], P + TX);
-                    dma(r1, xh6, lv + yrow[3 + r], P + TX + 6);
+                    dma(rfo, xn, qoff(0) + o, P);
+                    dma(rfo, xh, qoff(1) + o, P + TX);
+                    dma(rfo, xh6, qoff(1) + o, P + TX + 6);
                 } else {
-                    const auto r2 = popr(2);
-                    dma(r2, xh, lv + yrow[3 + r], P + TX + HX);
-                    dma(r2, xh6, lv + yrow[3 + r], P + TX + HX + 6);
+                    dma(rfo, xh, qoff(2) + o, P + TX + HX);
+                    dma(rfo, xh6, qoff(2) + o, P + TX + HX + 6);
                 }
             };
             {
@@ -356,8 +358,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                 if (adv) {
                     if constexpr (G == 0) {
                         const unsigned lv = SZ * (unsigned)x * (unsigned)plane;
-                        c5 = IO::ld(popr(5), own_b, lv, a.inv_scale[5]);
-                        c6 = IO::ld(popr(6), own_b, lv, a.inv_scale[6]);
+                        c5 = IO::ld(rfo, own_b, qoff(5) + lv, a.inv_scale[5]);
+                        c6 = IO::ld(rfo, own_b, qoff(6) + lv, a.inv_scale[6]);
                         asm volatile("" ::: "memory");  // keep them ahead of the staging loads
                     }
                     fetch(min(x + 1, L1), b ^ 1);  // the other buffer holds level x-1
@@ -475,7 +477,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         using cdouble = const __attribute__((address_space(4))) double;
         const float rho_mod = (float)*(cdouble*)a.rho_modify;
         const float force = (float)*(cdouble*)a.force;
-        const int fbytes = (int)(4u * (unsigned)N);
+        const unsigned fbytes = 4u * (unsigned)N;
+        const auto rfn = rsrc_of(a.fn, NQ * nbq);
+        const auto rrho = rsrc_of(a.rho, fbytes), ru = rsrc_of(a.u, fbytes), rv = rsrc_of(a.v, fbytes),
+                   rw = rsrc_of(a.w, fbytes);
         // directions 0, 1, 2 of the target (no z dependence), staged in LDS by the ring
         // waves: the node's own 0 and the 7 x taps of 1 and 2, reduced to F0, F1, F2
         auto dec = [&](float v, float isc) -> float {
@@ -521,12 +526,12 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                                 tau_eff);
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) IO::st(rsrc_of(a.fn + (long long)q * N, nbytes), own_b, lv, F[q], a.scale[q]);
+            for (int q = 0; q < NQ; ++q) IO::st(rfn, own_b, qoff(q) + lv, F[q], a.scale[q]);
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rsrc_of(a.rho, fbytes), (int)own4, (int)lf, kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), rsrc_of(a.u, fbytes), (int)own4, (int)lf, kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rsrc_of(a.v, fbytes), (int)own4, (int)lf, kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rsrc_of(a.w, fbytes), (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rrho, (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), ru, (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rv, (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rw, (int)own4, (int)lf, kStoreAux);
         };
         Sched sc{L0 - 1, k0};
         bool adv;
@@ -561,9 +566,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
 {
-    // buffer resources address one population or field array with 32-bit byte offsets
-    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && g.n * 4 < (1LL << 31) &&
-           (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 31);
+    // buffer resources address a population buffer (19 arrays), a field or the xi table
+    // with 32-bit byte offsets
+    const long long sz = store == Store::F32 ? 4 : 2;
+    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && NQ * g.n * sz < (1LL << 32) &&
+           g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
 }
 
 template <Store ST>

@@ -6,16 +6,16 @@
 //   ring waves (4): two direction groups x two y rows; a lane keeps a ring of its group's
 //       8 directions x 7 levels in registers
 //           group A: 5 6 7 8 9 10 11 12        group B: 3 4 13 14 15 16 17 18
-//   collision waves (2): one per y row; they take the 16 interpolated populations from
-//       LDS, read 0, 1, 2 themselves, apply the walls, collide (collide_pairs,
-//       evolution.h:597-679) and store.
+//   collision waves (2): one per y row; they take the 16 interpolated populations and
+//       the staged rows of 0, 1, 2 from LDS, apply the walls, collide (collide_pairs,
+//       evolution.h:597-679) and store; they issue no global load.
 // The roles run separate loops with one workgroup barrier per step, so each role's
 // registers are allocated for that role alone and the collision of one target overlaps
 // the ring work of the next.
 //
 // Per z level the ring waves stage the 14 populations with horizontal neighbours into LDS
 // (double-buffered, with a 3-node halo where the direction needs one: 7-10 in x and y,
-// 3, 4, 15-18 in y, 11-14 in x; 5 and 6 are read at the lane's own node), run the
+// 3, 4, 15-18 in y, 11-14 in x; 5 and 6 at the nodes only), run the
 // separable x and y Lagrange filters from LDS (group A runs the y pass of 7-10 for its own
 // row in a wave-private LDS area) and append the filtered values to the ring.  Once the
 // ring covers a target level's xi stencil, each ring wave applies the xi weights of the
@@ -49,7 +49,8 @@ constexpr int X_SZ = TY * HX;   // 140: x halo only
 constexpr int OFF_XY = 0;
 constexpr int OFF_Y = OFF_XY + 4 * XY_SZ;
 constexpr int OFF_X = OFF_Y + 6 * Y_SZ;
-constexpr int STAGE = OFF_X + 4 * X_SZ;  // 5872 staged values per level
+constexpr int OFF_Z = OFF_X + 4 * X_SZ;  // 5, 6 at the node: [direction][row][TX]
+constexpr int STAGE = OFF_Z + 2 * TY * TX;  // 6128 staged values per level
 constexpr int OFF_YP = 2 * STAGE;        // group A: y pass of 7-10 for its row, 4 x HX per wave
 constexpr int HO_SZ = 16 * TX * TY;      // hand-over: directions 3-18 x 128 columns
 constexpr int OFF_HO = OFF_YP + TY * 4 * HX;
@@ -57,7 +58,7 @@ constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps 
 // directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
 constexpr int P_ROW = TX + 2 * HX;
 constexpr int OFF_P012 = OFF_XIB + 2 * TY * 64;
-constexpr int LDS_FLOATS = OFF_P012 + 2 * TY * P_ROW;  // 17472 floats = 68 KB: two workgroups per CU
+constexpr int LDS_FLOATS = OFF_P012 + 2 * TY * P_ROW;  // 17984 floats = 70 KB: two workgroups per CU
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
 static_assert(TY == 2, "the fetch split below assumes two rows per tile");
@@ -216,7 +217,6 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         const int col = ty * TX + tx;
         const int j = j0 + ty;
         const int jw = j < ny ? j : j % ny;
-        const unsigned own_b = SZ * (unsigned)(jw * nx + iw);
         // Staging loads are buffer loads into LDS: a resource per population array, the
         // row offset in an SGPR and this lane's column offset in a VGPR that stays live for
         // the whole kernel (per-load 64-bit VGPR addresses would be recycled by the
@@ -272,6 +272,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                 for (int r = 0; r < 2; ++r)
                     dma(rfo, xn, oy + yrow[2 * w + r], L + OFF_Y + d * Y_SZ + (2 * w + r) * TX);
             }
+            if (w >= TY) {  // group B: 5 and 6 at the nodes of row w - TY, for group A's ring
+                dma(rfo, xn, qoff(5) + lv + yrow[3 + w - TY], L + OFF_Z + (w - TY) * TX);
+                dma(rfo, xn, qoff(6) + lv + yrow[3 + w - TY], L + OFF_Z + (TY + w - TY) * TX);
+            }
         };
         // 16-bit storage: decode this wave's share to FP32 deviations in place
         auto decode_region = [&](float* R, int n, float isc) {
@@ -291,6 +295,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
                 for (int d = 0; d < 6; ++d)
                     decode_region(L + OFF_Y + d * Y_SZ + 2 * w * TX, 2 * TX, a.inv_scale[qy(d)]);
+                if (w >= TY) {
+                    decode_region(L + OFF_Z + (w - TY) * TX, TX, a.inv_scale[5]);
+                    decode_region(L + OFF_Z + (TY + w - TY) * TX, TX, a.inv_scale[6]);
+                }
             }
         };
 
@@ -301,12 +309,10 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
             for (int d = 0; d < 8; ++d)
 #pragma unroll
                 for (int m = 0; m < 7; ++m) g[d][m] = 0.f;
-            // Vector-memory order per wave: in the step that appends level x, the loads of
-            // directions 5, 6 of level x (group A), then the staging loads of level x+1 (at
-            // the last level, level L1 again into the free buffer); steps that append
-            // nothing issue no vector-memory instruction.  Every appending step thus issues
-            // the same sequence, and the loads of 5, 6 are complete once at most the staging
-            // loads issued after them are outstanding.
+            // Every vector-memory instruction of a ring wave is a staging load into LDS: in
+            // the step that appends level x, those of level x+1 (at the last level, level L1
+            // again into the free buffer); group B also stages the xi weights and the rows of
+            // 0-2 of the next target.  Each step waits for all of them before its barrier.
             fetch(L0, 0);
             int b = 0;  // stage buffer of the next level to append
             int s = 0;  // step
@@ -314,27 +320,25 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
             bool adv;
             int ho;
             // xi weights of each step's target are staged into LDS one step ahead by the
-            // group A waves (one 64-lane load of the 56 weights of its row; the last 8 lanes
+            // group B waves (one 64-lane load of the 56 weights of its row; the last 8 lanes
             // read past the row and are not used)
             const auto rxi = rsrc_of(a.t.xiw, xi_bytes);
             auto fetch_xi = [&](int tk, int par) {
-                if constexpr (G == 0)  // an FP32 table whatever the population storage
+                if constexpr (G == 1)  // an FP32 table whatever the population storage
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(
                         rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + ty) * 64), 4,
                         (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
             };
-            // rows of directions 0, 1, 2 of target tk for the collision waves: waves 0, 1 stage
-            // 0 and 1 of row w, waves 2, 3 stage 2 of row w-2 (no z dependence)
+            // rows of directions 0, 1, 2 of target tk for the collision waves, staged by the
+            // group B wave of each row (no z dependence)
             auto fetch012 = [&](int tk, int par) {
-                const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
-                const int r = w % TY;
-                float* P = lds + OFF_P012 + (par * TY + r) * P_ROW;
-                const unsigned o = lv + yrow[3 + r];
-                if (w < TY) {
+                if constexpr (G == 1) {
+                    const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
+                    float* P = lds + OFF_P012 + (par * TY + ty) * P_ROW;
+                    const unsigned o = lv + yrow[3 + ty];
                     dma(rfo, xn, qoff(0) + o, P);
                     dma(rfo, xh, qoff(1) + o, P + TX);
                     dma(rfo, xh6, qoff(1) + o, P + TX + 6);
-                } else {
                     dma(rfo, xh, qoff(2) + o, P + TX + HX);
                     dma(rfo, xh6, qoff(2) + o, P + TX + HX + 6);
                 }
@@ -354,14 +358,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                 if (adv) decode(b);
                 lds_barrier();  // level x staged by all waves; level x-1 and hand-over s-2 no longer read
                 const float* S = lds + b * STAGE;  // level x (steps that append)
-                float c5 = 0.f, c6 = 0.f;          // directions 5, 6 of level x at the lane's own node
                 if (adv) {
-                    if constexpr (G == 0) {
-                        const unsigned lv = SZ * (unsigned)x * (unsigned)plane;
-                        c5 = IO::ld(rfo, own_b, qoff(5) + lv, a.inv_scale[5]);
-                        c6 = IO::ld(rfo, own_b, qoff(6) + lv, a.inv_scale[6]);
-                        asm volatile("" ::: "memory");  // keep them ahead of the staging loads
-                    }
                     fetch(min(x + 1, L1), b ^ 1);  // the other buffer holds level x-1
                     b ^= 1;
                 }
@@ -406,8 +403,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                                 YP[d * HX + c] = sum;
                             }
                         }
-                        g[0][6] = c5;
-                        g[1][6] = c6;
+                        g[0][6] = S[OFF_Z + ty * TX + tx];         // 5 at the node (no filter)
+                        g[1][6] = S[OFF_Z + (TY + ty) * TX + tx];  // 6
 #pragma unroll
                         for (int d = 0; d < 4; ++d)  // 7-10: x pass over the y-passed row
                             g[2 + d][6] = xf(YP + d * HX + tx, wx[xsel(7 + d)]);

@@ -43,22 +43,40 @@ constexpr int NRW = 2 * TY;  // ring waves
 constexpr int NCW = TY;      // collision waves
 constexpr int NTH = 64 * (NRW + NCW);
 constexpr int HX = TX + 6, HY = TY + 6;
-constexpr int XY_SZ = HY * HX;  // 560: tile of a diagonal direction (x and y halo)
-constexpr int Y_SZ = HY * TX;   // 512: y halo only
-constexpr int X_SZ = TY * HX;   // 140: x halo only
-constexpr int OFF_XY = 0;
-constexpr int OFF_Y = OFF_XY + 4 * XY_SZ;
-constexpr int OFF_X = OFF_Y + 6 * Y_SZ;
-constexpr int OFF_Z = OFF_X + 4 * X_SZ;  // 5, 6 at the node: [direction][row][TX]
-constexpr int STAGE = OFF_Z + 2 * TY * TX;  // 6128 staged values per level
-constexpr int OFF_YP = 2 * STAGE;        // group A: y pass of 7-10 for its row, 4 x HX per wave
-constexpr int HO_SZ = 16 * TX * TY;      // hand-over: directions 3-18 x 128 columns
-constexpr int OFF_HO = OFF_YP + TY * 4 * HX;
-constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps x TY rows x 64
-// directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
-constexpr int P_ROW = TX + 2 * HX;
-constexpr int OFF_P012 = OFF_XIB + 2 * TY * 64;
-constexpr int LDS_FLOATS = OFF_P012 + 2 * TY * P_ROW;  // 17984 floats = 70 KB: two workgroups per CU
+constexpr int Y_SZ = HY * TX;  // 512: y halo only
+constexpr int HO_SZ = 16 * TX * TY;  // hand-over: directions 3-18 x 128 columns
+// 16-byte staging (FP32 storage, nx % 4 == 0): a level is staged by NW16 dwordx4 LDS-DMA
+// loads of 64 lanes, lane l of load n filling granule 64 n + l (16 bytes at 4 (64 n + l))
+constexpr int NW16 = 25;
+
+// LDS layout.  W: rows with an x halo are 72 wide and start at x = i0 - 4 (whole 16-byte
+// granules); otherwise they are 70 wide from x = i0 - 3.  A staged level holds
+//   XY: 7-10 [4][HY][HXW]   X: 11-14 [4][TY][HXW]   Z: 5, 6 [2][TY][TX]   Y: 3 4 15-18 [6][HY][TX]
+// (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).
+template <bool W>
+struct Lay {
+    static constexpr int HXW = W ? 72 : HX;  // x-halo row stride
+    static constexpr int XO = W ? 1 : 0;     // column of x = i0 - 3
+    static constexpr int XY_SZ = HY * HXW;
+    static constexpr int X_SZ = TY * HXW;
+    static constexpr int OFF_XY = 0;
+    static constexpr int OFF_X = W ? 4 * XY_SZ : 4 * XY_SZ + 6 * Y_SZ;
+    static constexpr int OFF_Z = OFF_X + 4 * X_SZ;
+    static constexpr int OFF_Y = W ? OFF_Z + 2 * TY * TX : 4 * XY_SZ;
+    static constexpr int STAGE = W ? NW16 * 256 : OFF_Z + 2 * TY * TX;
+    static constexpr int OFF_YP = 2 * STAGE;  // group A: y pass of 7-10 for its row, 4 x HXW per wave
+    static constexpr int OFF_HO = OFF_YP + TY * 4 * HXW;
+    static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps x TY rows x 64
+    // directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
+    static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
+    static constexpr int OFF_P012 = OFF_XIB + 2 * TY * 64;
+    static constexpr int LDS_FLOATS = OFF_P012 + 2 * TY * P_ROW;
+    static_assert(!W || OFF_Y + 6 * Y_SZ <= STAGE, "16-byte staging overflows its level buffer");
+    static_assert(!W || (OFF_X % 4 == 0 && OFF_Z % 4 == 0 && OFF_Y % 4 == 0 && STAGE % 4 == 0 && OFF_P012 % 4 == 0 &&
+                         P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
+};
+// 70 KB (4-byte staging) or 73 KB (16-byte staging): two workgroups per CU
+static_assert(Lay<false>::LDS_FLOATS * 4 <= 80 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 80 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
 static_assert(TY == 2, "the fetch split below assumes two rows per tile");
@@ -174,16 +192,67 @@ __device__ inline bool sched_step(Sched& sc, int k1, int nz, bool& adv, int& ho)
 
 }  // namespace
 
-template <Store ST>
+// 16-byte buffer load into LDS (gfx950): lane l fills the 16 bytes at ldst + 16 l.  The
+// host pass checks the builtin's size operand against host target features, so the call
+// exists in the device pass only.
+__device__ inline void lds_dma16(__amdgpu_buffer_rsrc_t r, float* ldst, unsigned voff, unsigned soff)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)ldst, 16, (int)voff, (int)soff, 0,
+                                             0);
+#endif
+}
+
+// Byte offset of staging granule G (16-byte staging) for the tile at (i0, j0): the level's
+// offset goes into the load's scalar offset.  Granules past the staged level (padding of
+// the last load) repeat granule 0.
+__device__ inline unsigned granule_offset(int G, int i0, int j0, int nx, int ny, unsigned nbq)
+{
+    using LW = Lay<true>;
+    auto wx = [&](int x) { return ((x % nx) + nx) % nx; };
+    auto wy = [&](int y) { return ((y % ny) + ny) % ny; };
+    int q, y, x;
+    if (G >= (LW::OFF_Y + 6 * Y_SZ) / 4) G = 0;
+    if (G < LW::OFF_X / 4) {  // 7-10: 18 granules per row, rows j0-3 .. j0+4
+        const int row = G / 18, cg = G % 18;
+        q = 7 + row / HY;
+        y = wy(j0 - 3 + row % HY);
+        x = wx(i0 - 4 + 4 * cg);
+    } else if (G < LW::OFF_Z / 4) {  // 11-14: rows j0, j0+1
+        const int g = G - LW::OFF_X / 4, row = g / 18, cg = g % 18;
+        q = 11 + row / TY;
+        y = wy(j0 + row % TY);
+        x = wx(i0 - 4 + 4 * cg);
+    } else if (G < LW::OFF_Y / 4) {  // 5, 6 at the nodes
+        const int g = G - LW::OFF_Z / 4;
+        q = 5 + g / (TY * TX / 4);
+        y = wy(j0 + (g / (TX / 4)) % TY);
+        x = wx(i0 + 4 * (g % (TX / 4)));
+    } else {  // 3 4 15-18: rows j0-3 .. j0+4, no x halo
+        const int g = G - LW::OFF_Y / 4, row = g / (TX / 4);
+        q = qy(row / HY);
+        y = wy(j0 - 3 + row % HY);
+        x = wx(i0 + 4 * (g % (TX / 4)));
+    }
+    return (unsigned)q * nbq + 4u * (unsigned)(y * nx + x);
+}
+
+template <Store ST, bool W>
 __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled(StepArgs<ST> a,
                                                                                                  TiledGeom tg)
 {
     using CD = Codec<ST>;
     using raw = typename CD::raw;
     using IO = BufIO<ST>;
+    using LY = Lay<W>;
     constexpr bool k16 = (ST != Store::F32);
+    static_assert(!W || !k16, "16-byte staging is for FP32 storage");
     constexpr unsigned SZ = sizeof(raw);
-    __shared__ float lds[LDS_FLOATS];
+    constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
+    constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
+    constexpr int OFF_YP = LY::OFF_YP, OFF_HO = LY::OFF_HO, OFF_XIB = LY::OFF_XIB, OFF_P012 = LY::OFF_P012;
+    constexpr int P_ROW = LY::P_ROW;
+    __shared__ __attribute__((aligned(16))) float lds[LY::LDS_FLOATS];
 
     // XCD-aware work mapping: block b runs on XCD b % 8
     const int per_xcd = tg.nblocks / 8;
@@ -230,6 +299,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         unsigned yrow[HY];  // byte offset of staged row rr in a plane
 #pragma unroll
         for (int rr = 0; rr < HY; ++rr) yrow[rr] = SZ * (unsigned)(wrap1(j0 - 3 + rr, ny) * nx);
+        // 16-byte staging: this wave issues loads n = w, w + 4, ... (< NW16) of each level
+        unsigned g16[(NW16 + NRW - 1) / NRW];
+        if constexpr (W) {
+#pragma unroll
+            for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m)
+                g16[m] = granule_offset(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
+        }
         float wx[2][7], wy[2][7];
 #pragma unroll
         for (int l = 0; l < 7; ++l) {
@@ -251,6 +327,15 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         auto fetch = [&](int lev, int b) {
             const unsigned lv = SZ * (unsigned)lev * (unsigned)plane;
             float* L = lds + b * STAGE;
+            if constexpr (W) {
+#pragma unroll
+                for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m) {
+                    const int n = w + NRW * m;
+                    if (n < NW16)
+                        lds_dma16(rfo, L + 256 * n, g16[m], lv);
+                }
+                return;
+            }
             const unsigned od = qoff(7 + w) + lv;
 #pragma unroll
             for (int rr = 0; rr < HY; ++rr) {
@@ -323,6 +408,13 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
             // group B waves (one 64-lane load of the 56 weights of its row; the last 8 lanes
             // read past the row and are not used)
             const auto rxi = rsrc_of(a.t.xiw, xi_bytes);
+            unsigned p16 = 0;  // 16-byte staging of directions 0-2 (group B): this lane's granule
+            if constexpr (G == 1 && W) {
+                const int g = lane < 52 ? lane : 0;
+                const int q = g < 16 ? 0 : (g < 34 ? 1 : 2);
+                const int x = g < 16 ? i0 + 4 * g : i0 - 4 + 4 * (g - (q == 1 ? 16 : 34));
+                p16 = (unsigned)q * nbq + 4u * (unsigned)(jw * nx + ((x % nx) + nx) % nx);
+            }
             auto fetch_xi = [&](int tk, int par) {
                 if constexpr (G == 1)  // an FP32 table whatever the population storage
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -332,7 +424,11 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
             // rows of directions 0, 1, 2 of target tk for the collision waves, staged by the
             // group B wave of each row (no z dependence)
             auto fetch012 = [&](int tk, int par) {
-                if constexpr (G == 1) {
+                if constexpr (G == 1 && W) {
+                    // one 16-byte load: granules 0-15 direction 0, 16-33 direction 1 and 34-51
+                    // direction 2 (x halo from i0 - 4); the 12 padding lanes repeat granule 0
+                    lds_dma16(rfo, lds + OFF_P012 + (par * TY + ty) * P_ROW, p16, SZ * (unsigned)tk * (unsigned)plane);
+                } else if constexpr (G == 1) {
                     const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
                     float* P = lds + OFF_P012 + (par * TY + ty) * P_ROW;
                     const unsigned o = lv + yrow[3 + ty];
@@ -391,33 +487,33 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
                     if constexpr (G == 0) {
                         // y pass of 7-10 on this wave's row, all HX columns, into its own LDS
                         // area (read back by this wave only: no barrier needed)
-                        float* YP = lds + OFF_YP + ty * 4 * HX;
+                        float* YP = lds + OFF_YP + ty * 4 * HXW;
 #pragma unroll
                         for (int d = 0; d < 4; ++d) {
                             const float* wyy = wy[ysel(7 + d)];
-                            for (int c = lane; c < HX; c += 64) {
-                                const float* src = S + OFF_XY + d * XY_SZ + ty * HX + c;
+                            for (int c = lane; c < HXW; c += 64) {
+                                const float* src = S + OFF_XY + d * XY_SZ + ty * HXW + c;
                                 float sum = src[0] * wyy[0];
 #pragma unroll
-                                for (int n = 1; n < 7; ++n) sum = sum + src[n * HX] * wyy[n];
-                                YP[d * HX + c] = sum;
+                                for (int n = 1; n < 7; ++n) sum = sum + src[n * HXW] * wyy[n];
+                                YP[d * HXW + c] = sum;
                             }
                         }
                         g[0][6] = S[OFF_Z + ty * TX + tx];         // 5 at the node (no filter)
                         g[1][6] = S[OFF_Z + (TY + ty) * TX + tx];  // 6
 #pragma unroll
                         for (int d = 0; d < 4; ++d)  // 7-10: x pass over the y-passed row
-                            g[2 + d][6] = xf(YP + d * HX + tx, wx[xsel(7 + d)]);
+                            g[2 + d][6] = xf(YP + d * HXW + XO + tx, wx[xsel(7 + d)]);
 #pragma unroll
                         for (int d = 0; d < 2; ++d)  // 11, 12: x filter
-                            g[6 + d][6] = xf(S + OFF_X + d * X_SZ + ty * HX + tx, wx[xsel(11 + d)]);
+                            g[6 + d][6] = xf(S + OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]);
                     } else {
 #pragma unroll
                         for (int d = 0; d < 2; ++d)  // 3, 4: y filter
                             g[d][6] = yf(S + OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(3 + d)]);
 #pragma unroll
                         for (int d = 2; d < 4; ++d)  // 13, 14: x filter
-                            g[d][6] = xf(S + OFF_X + d * X_SZ + ty * HX + tx, wx[xsel(11 + d)]);
+                            g[d][6] = xf(S + OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]);
 #pragma unroll
                         for (int d = 2; d < 6; ++d)  // 15-18: y filter
                             g[2 + d][6] = yf(S + OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]);
@@ -490,8 +586,8 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
         };
         auto reduce012 = [&](float (&F)[NQ], const float* P) {
             F[0] = dec(P[tx], isc0);
-            const float* P1 = P + TX + tx;
-            const float* P2 = P + TX + HX + tx;
+            const float* P1 = P + TX + XO + tx;
+            const float* P2 = P + TX + HXW + XO + tx;
             float s1 = dec(P1[0], isc1) * wx0[0], s2 = dec(P2[0], isc2) * wx2[0];
 #pragma unroll
             for (int l = 1; l < 7; ++l) {
@@ -570,21 +666,28 @@ bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
            g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
 }
 
-template <Store ST>
+template <Store ST, bool W>
 static hipError_t launch_t(hipStream_t st, const StepLaunch& L)
 {
     StepArgs<ST> a = make_step_args<ST>(L);
     TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz);
-    hipLaunchKernelGGL(k_collide_tiled<ST>, dim3(tg.nblocks), dim3(NTH), 0, st, a, tg);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W>), dim3(tg.nblocks), dim3(NTH), 0, st, a, tg);
     return hipGetLastError();
+}
+
+bool tiled_wide_staging(const LaunchGeom& g, Store store)
+{
+    // whole 16-byte granules of a row never straddle the periodic wrap in x
+    return store == Store::F32 && g.nx % 4 == 0;
 }
 
 hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
 {
     switch (L.store) {
-    case Store::F32: return launch_t<Store::F32>(st, L);
-    case Store::BF16: return launch_t<Store::BF16>(st, L);
-    case Store::FP16: return launch_t<Store::FP16>(st, L);
+    case Store::F32:
+        return tiled_wide_staging(L.g, L.store) ? launch_t<Store::F32, true>(st, L) : launch_t<Store::F32, false>(st, L);
+    case Store::BF16: return launch_t<Store::BF16, false>(st, L);
+    case Store::FP16: return launch_t<Store::FP16, false>(st, L);
     default: return hipErrorNotSupported;
     }
 }

@@ -205,9 +205,12 @@ def test_kernel_is_native(pkg):
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
 @pytest.mark.parametrize("shape", [dict(nx=64, ny=32, nz=32), dict(nx=48, ny=20, nz=24), dict(nx=16, ny=32, nz=64),
-                                   dict(nx=64, ny=32, nz=48, cs_smag=0.1)])
+                                   dict(nx=64, ny=32, nz=48, cs_smag=0.1), dict(nx=20, ny=21, nz=24),
+                                   dict(nx=18, ny=20, nz=24)])
 def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
-    """Column-marching kernel (z chunks, wrapped partial tiles) against the FP64 oracle."""
+    """Column-marching kernel (z chunks, wrapped partial tiles) against the FP64 oracle.
+    FP32 storage with nx % 4 == 0 stages whole 16-byte granules (nx = 16, 20, 48 wrap inside
+    a tile; ny = 21 leaves a partial row tile); nx = 18 takes the 4-byte staging."""
     o, s = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED, **shape)
     assert s.kernel_in_use() == pkg.KERNEL_TILED
     for x in (o, s):

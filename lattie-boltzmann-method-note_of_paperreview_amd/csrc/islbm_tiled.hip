@@ -38,32 +38,40 @@ namespace islbm {
 
 namespace {
 
-constexpr int TX = 64, TY = 2;
-constexpr int NRW = 2 * TY;  // ring waves
-constexpr int NCW = TY;      // collision waves
-constexpr int NTH = 64 * (NRW + NCW);
-constexpr int HX = TX + 6, HY = TY + 6;
-constexpr int Y_SZ = HY * TX;  // 512: y halo only
-constexpr int HO_SZ = 16 * TX * TY;  // hand-over: directions 3-18 x 128 columns
-// 16-byte staging (FP32 storage, nx % 4 == 0): a level is staged by NW16 dwordx4 LDS-DMA
-// loads of 64 lanes, lane l of load n filling granule 64 n + l (16 bytes at 4 (64 n + l))
-constexpr int NW16 = 25;
+constexpr int TX = 64;
+constexpr int HX = TX + 6;
 
-// LDS layout.  W: rows with an x halo are 72 wide and start at x = i0 - 4 (whole 16-byte
-// granules); otherwise they are 70 wide from x = i0 - 3.  A staged level holds
+// Tile and LDS layout.  A workgroup owns a TX x TY tile of node columns and has 3 TY waves:
+// ring waves of group A (w = 0 .. TY-1, one per row), of group B (w = TY .. 2TY-1) and
+// collision waves (w = 2TY .. 3TY-1).  Waves w, w + 4, w + 8 share a SIMD.
+//   W = false (4-byte staging, TY = 2): rows with an x halo are 70 wide from x = i0 - 3.
+//   W = true (16-byte staging, FP32 storage with nx % 4 == 0, TY = 4): rows with an x halo
+//       are 72 wide from x = i0 - 4 (whole 16-byte granules); each SIMD then holds one
+//       wave of each role.  A level is staged by NW16 dwordx4 LDS-DMA loads of 64 lanes,
+//       lane l of load n filling granule 64 n + l (16 bytes at float 4 (64 n + l)).
+// A staged level holds
 //   XY: 7-10 [4][HY][HXW]   X: 11-14 [4][TY][HXW]   Z: 5, 6 [2][TY][TX]   Y: 3 4 15-18 [6][HY][TX]
 // (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).
 template <bool W>
 struct Lay {
-    static constexpr int HXW = W ? 72 : HX;  // x-halo row stride
-    static constexpr int XO = W ? 1 : 0;     // column of x = i0 - 3
+    static constexpr int TY = W ? 4 : 2;
+    static constexpr int NRW = 2 * TY;  // ring waves
+    static constexpr int NCW = TY;      // collision waves
+    static constexpr int NTH = 64 * (NRW + NCW);
+    static constexpr int HY = TY + 6;
+    static constexpr int Y_SZ = HY * TX;       // y halo only
+    static constexpr int HO_SZ = 16 * TX * TY;  // hand-over: directions 3-18 x TX TY columns
+    static constexpr int HXW = W ? 72 : HX;    // x-halo row stride
+    static constexpr int XO = W ? 1 : 0;       // column of x = i0 - 3
     static constexpr int XY_SZ = HY * HXW;
     static constexpr int X_SZ = TY * HXW;
     static constexpr int OFF_XY = 0;
     static constexpr int OFF_X = W ? 4 * XY_SZ : 4 * XY_SZ + 6 * Y_SZ;
     static constexpr int OFF_Z = OFF_X + 4 * X_SZ;
     static constexpr int OFF_Y = W ? OFF_Z + 2 * TY * TX : 4 * XY_SZ;
-    static constexpr int STAGE = W ? NW16 * 256 : OFF_Z + 2 * TY * TX;
+    static constexpr int USED = W ? OFF_Y + 6 * Y_SZ : OFF_Z + 2 * TY * TX;
+    static constexpr int NW16 = (USED / 4 + 63) / 64;
+    static constexpr int STAGE = W ? NW16 * 256 : USED;
     static constexpr int OFF_YP = 2 * STAGE;  // group A: y pass of 7-10 for its row, 4 x HXW per wave
     static constexpr int OFF_HO = OFF_YP + TY * 4 * HXW;
     static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps x TY rows x 64
@@ -71,15 +79,16 @@ struct Lay {
     static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
     static constexpr int OFF_P012 = OFF_XIB + 2 * TY * 64;
     static constexpr int LDS_FLOATS = OFF_P012 + 2 * TY * P_ROW;
-    static_assert(!W || OFF_Y + 6 * Y_SZ <= STAGE, "16-byte staging overflows its level buffer");
+    // workgroups resident at once on 256 CUs: the LDS-DMA staging runs at one per CU
+    static constexpr int RESIDENT = W ? 256 : 512;
     static_assert(!W || (OFF_X % 4 == 0 && OFF_Z % 4 == 0 && OFF_Y % 4 == 0 && STAGE % 4 == 0 && OFF_P012 % 4 == 0 &&
                          P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
+    static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
 };
-// 70 KB (4-byte staging) or 73 KB (16-byte staging): two workgroups per CU
-static_assert(Lay<false>::LDS_FLOATS * 4 <= 80 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 80 * 1024, "LDS budget");
+// 70 KB (4-byte staging, two per CU fit) or 113 KB (16-byte staging)
+static_assert(Lay<false>::LDS_FLOATS * 4 <= 80 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
-static_assert(TY == 2, "the fetch split below assumes two rows per tile");
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
 
 // staged direction of each y-halo LDS region
@@ -99,15 +108,15 @@ struct TiledGeom {
     int ntx, nty, ntiles, nchunks, kc, nwork, nblocks;
 };
 
-TiledGeom tiled_geom(int nx, int ny, int nz)
+TiledGeom tiled_geom(int nx, int ny, int nz, int ty, int resident)
 {
     TiledGeom t;
     t.ntx = (nx + TX - 1) / TX;
-    t.nty = (ny + TY - 1) / TY;
+    t.nty = (ny + ty - 1) / ty;
     t.ntiles = t.ntx * t.nty;
-    // two resident workgroups per CU on 256 CUs, z chunks of at least 16 levels (each
+    // enough z chunks to fill the resident workgroup slots, of at least 16 levels (each
     // chunk re-reads a 6-level xi halo)
-    int want = (512 + t.ntiles - 1) / t.ntiles;
+    int want = (resident + t.ntiles - 1) / t.ntiles;
     int maxc = nz / 16 > 0 ? nz / 16 : 1;
     t.nchunks = want < 1 ? 1 : (want > maxc ? maxc : want);
     t.kc = (nz + t.nchunks - 1) / t.nchunks;
@@ -209,6 +218,7 @@ __device__ inline void lds_dma16(__amdgpu_buffer_rsrc_t r, float* ldst, unsigned
 __device__ inline unsigned granule_offset(int G, int i0, int j0, int nx, int ny, unsigned nbq)
 {
     using LW = Lay<true>;
+    constexpr int TY = LW::TY, HY = LW::HY, Y_SZ = LW::Y_SZ;
     auto wx = [&](int x) { return ((x % nx) + nx) % nx; };
     auto wy = [&](int y) { return ((y % ny) + ny) % ny; };
     int q, y, x;
@@ -238,8 +248,8 @@ __device__ inline unsigned granule_offset(int G, int i0, int j0, int nx, int ny,
 }
 
 template <Store ST, bool W>
-__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled(StepArgs<ST> a,
-                                                                                                 TiledGeom tg)
+__global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled(StepArgs<ST> a,
+                                                                                                          TiledGeom tg)
 {
     using CD = Codec<ST>;
     using raw = typename CD::raw;
@@ -252,6 +262,7 @@ __global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(3))) void k
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
     constexpr int OFF_YP = LY::OFF_YP, OFF_HO = LY::OFF_HO, OFF_XIB = LY::OFF_XIB, OFF_P012 = LY::OFF_P012;
     constexpr int P_ROW = LY::P_ROW;
+    constexpr int TY = LY::TY, NRW = LY::NRW, HY = LY::HY, Y_SZ = LY::Y_SZ, HO_SZ = LY::HO_SZ, NW16 = LY::NW16;
     __shared__ __attribute__((aligned(16))) float lds[LY::LDS_FLOATS];
 
     // XCD-aware work mapping: block b runs on XCD b % 8
@@ -670,8 +681,8 @@ template <Store ST, bool W>
 static hipError_t launch_t(hipStream_t st, const StepLaunch& L)
 {
     StepArgs<ST> a = make_step_args<ST>(L);
-    TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W>), dim3(tg.nblocks), dim3(NTH), 0, st, a, tg);
+    TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz, Lay<W>::TY, Lay<W>::RESIDENT);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W>), dim3(tg.nblocks), dim3(Lay<W>::NTH), 0, st, a, tg);
     return hipGetLastError();
 }
 

@@ -130,7 +130,7 @@ namespace {
 
 void free_all(islbm_solver* s)
 {
-    void* ptrs[] = {s->f[0], s->f[1], s->mac[0], s->mac[1], s->mac[2], s->mac[3], s->xw0, s->xw2, s->yw0,
+    void* ptrs[] = {s->f[0], s->f[1], s->mac[0], s->xw0, s->xw2, s->yw0,
                     s->yw2, s->xiw, s->bq, s->bw, s->les, s->bflag, s->fluxw, s->scal};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -411,10 +411,10 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
         if (hipMalloc(&s->f[b], fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc f (%zu bytes)", fb));
         if (hipMemset(s->f[b], 0, fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset f"));
     }
-    for (int a = 0; a < 4; ++a) {
-        if (hipMalloc(&s->mac[a], mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc macros"));
-        if (hipMemset(s->mac[a], 0, mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset macros"));
-    }
+    // rho, u, v, w in one allocation, [4][nz][ny][nx]
+    if (hipMalloc(&s->mac[0], 4 * mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc macros"));
+    if (hipMemset(s->mac[0], 0, 4 * mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset macros"));
+    for (int a = 1; a < 4; ++a) s->mac[a] = (char*)s->mac[0] + a * mb;
     if (hipMalloc((void**)&s->scal, 8 * sizeof(double)) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc scalars"));
     const double sc[8] = {s->g.force0, 0.0, 0.0, 0, 0, 0, 0, 0};
     if (hipMemcpy(s->scal, sc, sizeof sc, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "scalars"));

@@ -534,9 +534,9 @@ __host__ __device__ inline void collide_pairs(C (&F)[NQ], C rho_modify, C dt, C 
     // r_a = s_a m_a / |M_a|^2
     const C r1 = m1 * C(1.19 / 2394.0), r2 = m2 * C(1.4 / 252.0);
     const C r4 = m4 * C(1.2 / 40.0), r6 = m6 * C(1.2 / 40.0), r8 = m8 * C(1.2 / 40.0);
-    const C r9 = m9 * (s_nu / C(36)), r10 = m10 * C(1.4 / 72.0);
-    const C r11 = m11 * (s_nu / C(12)), r12 = m12 * C(1.4 / 24.0);
-    const C r13 = m13 * (s_nu / C(4)), r14 = m14 * (s_nu / C(4)), r15 = m15 * (s_nu / C(4));
+    const C r9 = m9 * (s_nu * C(1.0 / 36.0)), r10 = m10 * C(1.4 / 72.0);
+    const C r11 = m11 * (s_nu * C(1.0 / 12.0)), r12 = m12 * C(1.4 / 24.0);
+    const C r13 = m13 * (s_nu * C(0.25)), r14 = m14 * (s_nu * C(0.25)), r15 = m15 * (s_nu * C(0.25));
     const C r16 = m16 * C(1.5 / 8.0), r17 = m17 * C(1.5 / 8.0), r18 = m18 * C(1.5 / 8.0);
 
     // back-transform: even part E per pair class, odd part O per pair

@@ -583,8 +583,8 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         const float force = (float)*(cdouble*)a.force;
         const unsigned fbytes = 4u * (unsigned)N;
         const auto rfn = rsrc_of(a.fn, NQ * nbq);
-        const auto rrho = rsrc_of(a.rho, fbytes), ru = rsrc_of(a.u, fbytes), rv = rsrc_of(a.v, fbytes),
-                   rw = rsrc_of(a.w, fbytes);
+        // rho, u, v, w are consecutive arrays of one allocation (checked at launch)
+        const auto rmac = rsrc_of(a.rho, 4 * fbytes);
         // directions 0, 1, 2 of the target (no z dependence), staged in LDS by the ring
         // waves: the node's own 0 and the 7 x taps of 1 and 2, reduced to F0, F1, F2
         auto dec = [&](float v, float isc) -> float {
@@ -632,10 +632,10 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
 #pragma unroll
             for (int q = 0; q < NQ; ++q) IO::st(rfn, own_b, qoff(q) + lv, F[q], a.scale[q]);
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rrho, (int)own4, (int)lf, kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), ru, (int)own4, (int)lf, kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rv, (int)own4, (int)lf, kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rw, (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rmac, (int)own4, (int)lf, kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), rmac, (int)own4, (int)(lf + fbytes), kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rmac, (int)own4, (int)(lf + 2 * fbytes), kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rmac, (int)own4, (int)(lf + 3 * fbytes), kStoreAux);
         };
         Sched sc{L0 - 1, k0};
         bool adv;
@@ -674,7 +674,7 @@ bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
     // with 32-bit byte offsets
     const long long sz = store == Store::F32 ? 4 : 2;
     return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && NQ * g.n * sz < (1LL << 32) &&
-           g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
+           4 * g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
 }
 
 template <Store ST, bool W>
@@ -694,6 +694,11 @@ bool tiled_wide_staging(const LaunchGeom& g, Store store)
 
 hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
 {
+    // the kernel addresses rho, u, v, w through one buffer resource
+    const char* r = (const char*)L.rho;
+    const size_t fb = (size_t)L.g.n * 4;
+    if ((const char*)L.u != r + fb || (const char*)L.v != r + 2 * fb || (const char*)L.w != r + 3 * fb)
+        return hipErrorInvalidValue;
     switch (L.store) {
     case Store::F32:
         return tiled_wide_staging(L.g, L.store) ? launch_t<Store::F32, true>(st, L) : launch_t<Store::F32, false>(st, L);

@@ -72,7 +72,9 @@ struct Lay {
     static constexpr int USED = W ? OFF_Y + 6 * Y_SZ : OFF_Z + 2 * TY * TX;
     static constexpr int NW16 = (USED / 4 + 63) / 64;
     static constexpr int STAGE = W ? NW16 * 256 : USED;
-    static constexpr int OFF_YP = 2 * STAGE;  // group A: y pass of 7-10 for its row, 4 x HXW per wave
+    // staged levels in LDS: a level's loads are issued two steps before it is appended
+    static constexpr int NBUF = 3;
+    static constexpr int OFF_YP = NBUF * STAGE;  // group A: y pass of 7-10 for its row, 4 x HXW per wave
     static constexpr int OFF_HO = OFF_YP + TY * 4 * HXW;
     static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps x TY rows x 64
     // directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
@@ -85,8 +87,8 @@ struct Lay {
                          P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
 };
-// 70 KB (4-byte staging, two per CU fit) or 113 KB (16-byte staging)
-static_assert(Lay<false>::LDS_FLOATS * 4 <= 80 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
+// 94 KB (4-byte staging) or 146 KB (16-byte staging): one workgroup per CU
+static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
     constexpr int OFF_YP = LY::OFF_YP, OFF_HO = LY::OFF_HO, OFF_XIB = LY::OFF_XIB, OFF_P012 = LY::OFF_P012;
-    constexpr int P_ROW = LY::P_ROW;
+    constexpr int P_ROW = LY::P_ROW, NBUF = LY::NBUF;
     constexpr int TY = LY::TY, NRW = LY::NRW, HY = LY::HY, Y_SZ = LY::Y_SZ, HO_SZ = LY::HO_SZ, NW16 = LY::NW16;
     __shared__ __attribute__((aligned(16))) float lds[LY::LDS_FLOATS];
 
@@ -405,12 +407,26 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             for (int d = 0; d < 8; ++d)
 #pragma unroll
                 for (int m = 0; m < 7; ++m) g[d][m] = 0.f;
-            // Every vector-memory instruction of a ring wave is a staging load into LDS: in
-            // the step that appends level x, those of level x+1 (at the last level, level L1
-            // again into the free buffer); group B also stages the xi weights and the rows of
-            // 0-2 of the next target.  Each step waits for all of them before its barrier.
+            // Every vector-memory instruction of a ring wave is a staging load into LDS.  In a
+            // step, group B first stages the xi weights and the rows of 0-2 of the next target
+            // (read in the next step), then every ring wave issues its share of the level two
+            // above the one it appends (at the top of the column, level L1 again into the free
+            // buffer).  Loads complete in order, so a step waits for all but the last level
+            // batch (vmcnt = its load count) when the step before issued one, else for all.
+            const int nbatch = W ? (NW16 - w + NRW - 1) / NRW : (G == 0 ? 32 : 34);
+            auto wait_staged = [&](bool batch_last) {
+                constexpr int hi = W ? (NW16 + NRW - 1) / NRW : 34, lo = W ? NW16 / NRW : 32;
+                if (!batch_last)
+                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+                else if (nbatch == hi)
+                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(hi));
+                else if (nbatch == lo)
+                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(lo));
+                else
+                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+            };
             fetch(L0, 0);
-            int b = 0;  // stage buffer of the next level to append
+            int b = 0;  // stage buffer of the next level to append (level L0 + a in buffer a % NBUF)
             int s = 0;  // step
             Sched sc{L0 - 1, k0};
             bool adv;
@@ -457,18 +473,16 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                 sched_step(nx0, k1, nz, a0, h0);
                 fetch_xi(h0 >= 0 ? h0 : k0, 0);
             }
+            fetch(min(L0 + 1, L1), 1);
+            bool batch_last = true;
             while (sched_step(sc, k1, nz, adv, ho)) {
                 const int x = sc.top;  // level appended in this step (if adv)
                 // this wave's share of level x and of the rows of directions 0-2 and xi weights
                 // staged for this step are in LDS
-                __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+                wait_staged(batch_last);
                 if (adv) decode(b);
                 lds_barrier();  // level x staged by all waves; level x-1 and hand-over s-2 no longer read
                 const float* S = lds + b * STAGE;  // level x (steps that append)
-                if (adv) {
-                    fetch(min(x + 1, L1), b ^ 1);  // the other buffer holds level x-1
-                    b ^= 1;
-                }
                 {
                     Sched nx1 = sc;
                     bool a1;
@@ -477,6 +491,13 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                     fetch_xi(h1 >= 0 ? h1 : k0, (s + 1) & 1);  // read by step s+1 after its barrier
                 }
                 if (ho >= 0) fetch012(ho, s & 1);  // read by the collision waves in step s+1
+                if (adv) {
+                    // the buffer after the next holds level x-1, filtered in the last append step
+                    const int bn = b + 2 >= NBUF ? b + 2 - NBUF : b + 2;
+                    fetch(min(x + 2, L1), bn);
+                    b = b + 1 == NBUF ? 0 : b + 1;
+                }
+                batch_last = adv;
                 if (adv) {
                     // horizontal filters of this group's directions; shift the ring and append
                     auto xf = [&](const float* src, const float* wv) {

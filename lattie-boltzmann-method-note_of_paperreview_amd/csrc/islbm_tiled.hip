@@ -76,18 +76,19 @@ struct Lay {
     static constexpr int NBUF = 3;
     static constexpr int OFF_YP = NBUF * STAGE;  // group A: y pass of 7-10 for its row, 4 x HXW per wave
     static constexpr int OFF_HO = OFF_YP + TY * 4 * HXW;
-    static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, 2 steps x TY rows x 64
+    // per-target tables staged two steps ahead, NBUF slots (slot = hand-over step % NBUF)
+    static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, NBUF x TY rows x 64
     // directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
     static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
-    static constexpr int OFF_P012 = OFF_XIB + 2 * TY * 64;
-    static constexpr int LDS_FLOATS = OFF_P012 + 2 * TY * P_ROW;
+    static constexpr int OFF_P012 = OFF_XIB + NBUF * TY * 64;
+    static constexpr int LDS_FLOATS = OFF_P012 + NBUF * TY * P_ROW;
     // workgroups resident at once on 256 CUs: the LDS-DMA staging runs at one per CU
     static constexpr int RESIDENT = W ? 256 : 512;
     static_assert(!W || (OFF_X % 4 == 0 && OFF_Z % 4 == 0 && OFF_Y % 4 == 0 && STAGE % 4 == 0 && OFF_P012 % 4 == 0 &&
                          P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
 };
-// 94 KB (4-byte staging) or 146 KB (16-byte staging): one workgroup per CU
+// 96 KB (4-byte staging) or 151 KB (16-byte staging): one workgroup per CU
 static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
@@ -407,23 +408,28 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             for (int d = 0; d < 8; ++d)
 #pragma unroll
                 for (int m = 0; m < 7; ++m) g[d][m] = 0.f;
-            // Every vector-memory instruction of a ring wave is a staging load into LDS.  In a
-            // step, group B first stages the xi weights and the rows of 0-2 of the next target
-            // (read in the next step), then every ring wave issues its share of the level two
-            // above the one it appends (at the top of the column, level L1 again into the free
-            // buffer).  Loads complete in order, so a step waits for all but the last level
-            // batch (vmcnt = its load count) when the step before issued one, else for all.
+            // Every vector-memory instruction of a ring wave is a staging load into LDS, issued
+            // two steps before it is read.  In step s, group B stages the xi weights of the
+            // target handed over in step s+2 and the rows of 0-2 of the target handed over in
+            // step s+1 (the collision waves read them in step s+2), then every ring wave issues
+            // its share of the level two above the one it appends (at the top of the column,
+            // level L1 again into the free buffer).  Loads complete in order, so a step waits
+            // until only the loads of the step before are outstanding (vmcnt = their count).
             const int nbatch = W ? (NW16 - w + NRW - 1) / NRW : (G == 0 ? 32 : 34);
-            auto wait_staged = [&](bool batch_last) {
-                constexpr int hi = W ? (NW16 + NRW - 1) / NRW : 34, lo = W ? NW16 / NRW : 32;
-                if (!batch_last)
-                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-                else if (nbatch == hi)
-                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(hi));
-                else if (nbatch == lo)
-                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(lo));
-                else
-                    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+            constexpr int nP012 = W ? 1 : 5;
+            constexpr int ntab = G == 1 ? 1 + nP012 : 0;  // xi + rows of 0-2, every step
+            auto wait_staged = [&](int outstanding) {
+                // the counts a ring wave can leave outstanding (0 waits for everything)
+                switch (outstanding) {
+                case 2: __builtin_amdgcn_s_waitcnt(vmcnt_imm(2)); break;
+                case 4: __builtin_amdgcn_s_waitcnt(vmcnt_imm(4)); break;
+                case 5: __builtin_amdgcn_s_waitcnt(vmcnt_imm(5)); break;
+                case 6: __builtin_amdgcn_s_waitcnt(vmcnt_imm(6)); break;
+                case 7: __builtin_amdgcn_s_waitcnt(vmcnt_imm(7)); break;
+                case 32: __builtin_amdgcn_s_waitcnt(vmcnt_imm(32)); break;
+                case 40: __builtin_amdgcn_s_waitcnt(vmcnt_imm(40)); break;
+                default: __builtin_amdgcn_s_waitcnt(vmcnt_imm(0)); break;
+                }
             };
             fetch(L0, 0);
             int b = 0;  // stage buffer of the next level to append (level L0 + a in buffer a % NBUF)
@@ -466,38 +472,52 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                     dma(rfo, xh6, qoff(2) + o, P + TX + HX + 6);
                 }
             };
+            // hand-over targets of the next two steps (k0 where a step hands nothing over: the
+            // load keeps the per-step count fixed and fills a slot nobody reads)
+            auto lookahead = [&](const Sched& from, int& h1, int& h2) {
+                Sched n = from;
+                bool a_;
+                sched_step(n, k1, nz, a_, h1);
+                sched_step(n, k1, nz, a_, h2);
+                h1 = h1 >= 0 ? h1 : k0;
+                h2 = h2 >= 0 ? h2 : k0;
+            };
             {
-                Sched nx0 = sc;
-                bool a0;
-                int h0;
-                sched_step(nx0, k1, nz, a0, h0);
+                // the two steps before step 0: level L0 and the xi weights of step 0, then the
+                // xi weights of step 1, the rows of 0-2 of step 0 and level L0 + 1
+                Sched n = sc;
+                bool a_;
+                int h0, h1;
+                sched_step(n, k1, nz, a_, h0);
+                sched_step(n, k1, nz, a_, h1);
                 fetch_xi(h0 >= 0 ? h0 : k0, 0);
+                fetch_xi(h1 >= 0 ? h1 : k0, 1);
+                fetch012(h0 >= 0 ? h0 : k0, 0);
             }
             fetch(min(L0 + 1, L1), 1);
-            bool batch_last = true;
+            int outstanding = ntab + nbatch;  // loads issued in the step before
             while (sched_step(sc, k1, nz, adv, ho)) {
                 const int x = sc.top;  // level appended in this step (if adv)
                 // this wave's share of level x and of the rows of directions 0-2 and xi weights
                 // staged for this step are in LDS
-                wait_staged(batch_last);
+                wait_staged(outstanding);
                 if (adv) decode(b);
                 lds_barrier();  // level x staged by all waves; level x-1 and hand-over s-2 no longer read
                 const float* S = lds + b * STAGE;  // level x (steps that append)
                 {
-                    Sched nx1 = sc;
-                    bool a1;
-                    int h1;
-                    sched_step(nx1, k1, nz, a1, h1);
-                    fetch_xi(h1 >= 0 ? h1 : k0, (s + 1) & 1);  // read by step s+1 after its barrier
+                    int h1, h2;
+                    lookahead(sc, h1, h2);
+                    const int s1 = (s + 1) % NBUF, s2 = (s + 2) % NBUF;
+                    fetch_xi(h2, s2);   // read by the ring waves in step s+2
+                    fetch012(h1, s1);   // read by the collision waves in step s+2
                 }
-                if (ho >= 0) fetch012(ho, s & 1);  // read by the collision waves in step s+1
                 if (adv) {
                     // the buffer after the next holds level x-1, filtered in the last append step
                     const int bn = b + 2 >= NBUF ? b + 2 - NBUF : b + 2;
                     fetch(min(x + 2, L1), bn);
                     b = b + 1 == NBUF ? 0 : b + 1;
                 }
-                batch_last = adv;
+                outstanding = ntab + (adv ? nbatch : 0);
                 if (adv) {
                     // horizontal filters of this group's directions; shift the ring and append
                     auto xf = [&](const float* src, const float* wv) {
@@ -553,7 +573,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                 }
                 if (ho >= 0) {
                     // xi weights of the target (wave-uniform LDS reads, staged in step s-1)
-                    const float* XI = lds + OFF_XIB + ((s & 1) * TY + ty) * 64;
+                    const float* XI = lds + OFF_XIB + ((s % NBUF) * TY + ty) * 64;
                     float* HO = lds + OFF_HO + (s & 1) * HO_SZ;
 #pragma unroll
                     for (int d = 0; d < 8; ++d) {
@@ -680,7 +700,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             const float* HO = lds + OFF_HO + ((s - 1) & 1) * HO_SZ;
 #pragma unroll
             for (int q = 3; q < NQ; ++q) F[q] = HO[(q - 3) * (TX * TY) + col];
-            reduce012(F, lds + OFF_P012 + (((s - 1) & 1) * TY + ty) * P_ROW);
+            reduce012(F, lds + OFF_P012 + (((s - 1) % NBUF) * TY + ty) * P_ROW);
             collide_store(F, pend);
             if (!more) break;  // final step: the ring has appended and handed over everything
             pend = ho;

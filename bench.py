@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 # rho, u, v, w stores of every step (evolution.h:657-679)
 BYTES_PER_LUP = {"fp64": 38 * 8 + 4 * 8, "fp32": 38 * 4 + 4 * 4, "bf16": 38 * 2 + 4 * 4, "fp16": 38 * 2 + 4 * 4}
 HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS + % HBM roofline (FP32 vs FP64 @ stated err); serving p50/p99 solve latency"
 
 GRID = dict(nx=128, ny=256, nz=128)
 PHYSICS = {
@@ -74,6 +75,28 @@ def cpu_baseline(physics: dict, steps: int):
             "sample": f"FP64 oracle, {steps} steps of the same 128x256x128 workload ({dt:.1f} s)"}
 
 
+def accuracy_vs_fp64(pkg, prec: str, physics: dict, steps: int):
+    """Relative L2 error of rho and (u, v, w) of the measured precision against the FP64
+    kernel (equal to the FP64 oracle to 1e-10 on the parity grids) after `steps` steps on
+    the same seeded field, u = (0, U_ref, 0) + a 1e-3 perturbation, at the bench shape."""
+    import numpy as np
+
+    out = {}
+    for name in ("fp64", prec):
+        s = pkg.Solver(pkg.default_params(**GRID, **physics, precision=name,
+                                          kernel=pkg.KERNEL_PERNODE if name == "fp64" else pkg.KERNEL_AUTO))
+        s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+        s.step(steps)
+        out[name] = s.macros()
+        s.close()
+    r0, u0, v0, w0 = out["fp64"]
+    r1, u1, v1, w1 = out[prec]
+    e_u = float(np.sqrt(np.sum((u1 - u0) ** 2 + (v1 - v0) ** 2 + (w1 - w0) ** 2) / np.sum(u0 ** 2 + v0 ** 2 + w0 ** 2)))
+    e_r = float(np.linalg.norm(r1 - r0) / np.linalg.norm(r0))
+    return {"steps": steps, "rel_l2_u": e_u, "rel_l2_rho": e_r, "bound_u": {"fp32": 1e-5, "fp16": 6e-3, "bf16": 2e-2,
+                                                                          "fp64": 1e-10}[prec]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,8 +105,10 @@ def main() -> int:
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
     ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled"])
     ap.add_argument("--physics", default="stable-les", choices=sorted(PHYSICS))
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures the one-GPU path only; this build has no multi-GPU decomposition "
@@ -116,7 +141,7 @@ def main() -> int:
     ktag = "tiled" if s.kernel_in_use() == pkg.KERNEL_TILED else "pernode"
     traffic = traffic_from_profiles(ktag, args.precision)
     out = {
-        "metric": "MLUPS", "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
                                                           "fp16": "f16-storage/f32"}[args.precision],
@@ -134,6 +159,8 @@ def main() -> int:
         out["roofline"]["copy_gbs_measured"] = round(pkg.copy_bandwidth(1 << 30, 20), 1)
     except Exception as e:  # diagnostic only
         print(f"[bench] copy probe failed: {e}", file=sys.stderr)
+    if not args.no_accuracy and args.precision != "fp64":
+        out["accuracy_vs_fp64"] = accuracy_vs_fp64(pkg, args.precision, phys, args.accuracy_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(phys, args.cpu_steps)
     print(json.dumps(out))

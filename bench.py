@@ -97,6 +97,24 @@ def accuracy_vs_fp64(pkg, prec: str, physics: dict, steps: int):
                                                                           "fp64": 1e-10}[prec]}
 
 
+def serving(pkg, nreq: int, nsteps: int):
+    """Continuous-batching loop on a bounded sample of the serving workload (Poisson
+    arrivals at 1.5x the offered load of a 15 GLUPS service rate), beside one solve at a
+    time on the GPU in FP64 (per-node kernel) and in FP32 on the first requests."""
+    sv = importlib.import_module(PKG + ".serving")
+    reqs = sv.workload(nreq, nsteps=nsteps)
+    rate = sv.offered_rate(reqs, 15.0)
+    out = sv.run_batched(reqs, rate)
+    short = [dict(r, nsteps=min(200, nsteps)) for r in reqs[:3]]
+    seq64 = sv.run_sequential(short, precision="fp64")
+    seq32 = sv.run_sequential(short, precision="fp32", kernel=pkg.KERNEL_AUTO)
+    out.update({"workload": f"{nreq} requests x {nsteps} FP32 steps, Poisson arrivals {rate:.2f}/s",
+                "sequential_fp64_mlups": round(seq64["aggregate_mlups"], 1),
+                "sequential_fp32_mlups": round(seq32["aggregate_mlups"], 1),
+                "speedup_vs_sequential_fp64": round(out["aggregate_mlups"] / seq64["aggregate_mlups"], 2)})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +127,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-steps", type=int, default=1000)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures the one-GPU path only; this build has no multi-GPU decomposition "
@@ -159,6 +180,8 @@ def main() -> int:
         out["roofline"]["copy_gbs_measured"] = round(pkg.copy_bandwidth(1 << 30, 20), 1)
     except Exception as e:  # diagnostic only
         print(f"[bench] copy probe failed: {e}", file=sys.stderr)
+    if not args.no_serving:
+        out["serving"] = serving(pkg, args.serving_requests, args.serving_steps)
     if not args.no_accuracy and args.precision != "fp64":
         out["accuracy_vs_fp64"] = accuracy_vs_fp64(pkg, args.precision, phys, args.accuracy_steps)
     if not args.no_cpu_baseline:

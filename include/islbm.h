@@ -104,6 +104,15 @@ int islbm_download_macros(islbm_solver* s, double* rho, double* u, double* v, do
  * Work is queued on the solver's stream; islbm_sync waits for it. */
 int islbm_step(islbm_solver* s, int32_t nsteps);
 int islbm_sync(islbm_solver* s);
+/* islbm_step through a HIP graph: the first call with a chunk length records the chunk's
+ * launches (flux probe + stream-collide per step) into a graph, later calls of the same
+ * length replay it with one launch.  nsteps must be even (the population buffers
+ * alternate); a chunk that contains a forcing update is stepped as islbm_step does.
+ * No counterpart in the reference (its step is launched kernel by kernel). */
+int islbm_step_graph(islbm_solver* s, int32_t nsteps);
+/* The solver's HIP stream (hipStream_t) and the device bytes it holds. */
+int islbm_stream(islbm_solver* s, void** stream);
+int islbm_device_bytes(islbm_solver* s, size_t* bytes);
 
 /* Launch_ModifyForcingTerm (evolution.h:1003-1042) now, on the device. */
 int islbm_modify_forcing(islbm_solver* s);
@@ -132,6 +141,44 @@ int islbm_get_table(islbm_solver* s, const char* name, double* out, int32_t cap)
 int islbm_time_steps(islbm_solver* s, int32_t nsteps, double out[2]);
 
 const char* islbm_last_error(void);
+
+/* ---------------------------------------------------------------------------------- */
+/* Continuous-batching serving loop.  No counterpart in the reference, whose (absent)   */
+/* main runs one solve per program; it serves callers that run many periodic-hill       */
+/* solves of different grids, Reynolds numbers and hill steepnesses on one GPU.         */
+/* A request is one solve (params, seeded equilibrium start, step count).  Requests are  */
+/* admitted in submission order while the device-memory budget (islbm_required_bytes    */
+/* of every admitted solve) and the slot count allow; an admitted solve stays resident,  */
+/* runs on its own stream in chunks replayed from a HIP graph (islbm_step_graph, two    */
+/* chunks in flight) and is retired when done, returning its memory to the budget.      */
+/* One batch is driven from one host thread.                                             */
+/* ---------------------------------------------------------------------------------- */
+#define ISLBM_REQ_QUEUED 0
+#define ISLBM_REQ_ACTIVE 1
+#define ISLBM_REQ_DONE 2
+#define ISLBM_REQ_FAILED 3
+typedef struct islbm_batch islbm_batch;
+/* hbm_budget 0 = 90% of the device's free memory; chunk = steps per graph replay (even). */
+int islbm_batch_create(int32_t device, size_t hbm_budget, int32_t max_active, int32_t chunk, islbm_batch** out);
+int islbm_batch_destroy(islbm_batch* b);
+/* Queues a solve: f = f_eq(1, u0 + delta) as islbm_init_equilibrium, then nsteps steps.
+ * keep_fields != 0 keeps rho, u, v, w for islbm_batch_result.  A solve larger than the
+ * whole budget is marked ISLBM_REQ_FAILED at once. */
+int islbm_batch_submit(islbm_batch* b, const islbm_params* p, const double u0[3], double amp, uint64_t seed,
+                       int64_t nsteps, int32_t keep_fields, int64_t* id);
+/* One pass of the loop: retire finished solves, admit queued ones that fit, queue the next
+ * chunks of every active solve; with wait != 0 and nothing retired or admitted, first block
+ * until a chunk completes.  counts = [queued, active, done, failed] (may be NULL). */
+int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t counts[4]);
+/* state = ISLBM_REQ_*; times = [submit, admit, done] on the islbm_batch_now clock (s). */
+int islbm_batch_status(islbm_batch* b, int64_t id, int32_t* state, double times[3], int64_t* steps_done);
+/* Fields of a finished request submitted with keep_fields (released by the call). */
+int islbm_batch_result(islbm_batch* b, int64_t id, double* rho, double* u, double* v, double* w);
+/* [nodes, mean rho, max |u|, mean v] of a finished request. */
+int islbm_batch_summary(islbm_batch* b, int64_t id, double out[4]);
+int islbm_batch_bytes_in_use(islbm_batch* b, size_t* bytes);
+const char* islbm_batch_last_error(void);
+double islbm_batch_now(void);
 
 /* Bandwidth of a plain float4 copy kernel over `bytes` (read + write counted), averaged
  * over `iters` launches: the measured HBM reference beside the vendor peak. */

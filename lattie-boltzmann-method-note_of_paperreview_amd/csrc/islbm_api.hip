@@ -124,6 +124,8 @@ struct islbm_solver {
     bool auto_force = true;  // run the forcing controller every ndtfrc steps
     size_t bytes = 0;
     std::vector<hipEvent_t> evpool;
+    hipGraphExec_t graph = nullptr;  // islbm_step_graph: the captured chunk
+    int graph_steps = 0;
 };
 
 namespace {
@@ -135,6 +137,7 @@ void free_all(islbm_solver* s)
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -523,6 +526,64 @@ int islbm_step(islbm_solver* s, int32_t nsteps)
         int rc = enqueue_step(s, nullptr, nullptr);
         if (rc) return rc;
     }
+    return ISLBM_OK;
+}
+
+int islbm_step_graph(islbm_solver* s, int32_t nsteps)
+{
+    if (!s || nsteps <= 0 || nsteps % 2) return fail(ISLBM_ERR_ARG, "islbm_step_graph: nsteps must be even and positive");
+    HIPCHK(hipSetDevice(s->p.device));
+    // a forcing update inside the chunk is tied to the absolute step count: step eagerly
+    if (s->auto_force && (s->steps + nsteps) / s->p.ndtfrc > s->steps / s->p.ndtfrc) return islbm_step(s, nsteps);
+    if (!s->graph || s->graph_steps != nsteps) {
+        if (s->graph) {
+            HIPCHK(hipGraphExecDestroy(s->graph));
+            s->graph = nullptr;
+            s->graph_steps = 0;
+        }
+        // record the chunk without running it; the buffer parity returns to cur after an
+        // even number of steps, so every replay starts from the same buffer
+        const long long steps0 = s->steps;
+        const int cur0 = s->cur;
+        const bool af = s->auto_force;
+        s->auto_force = false;
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int rc = ISLBM_OK;
+        for (int t = 0; t < nsteps && rc == ISLBM_OK; ++t) rc = enqueue_step(s, nullptr, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->stream, &g);
+        s->auto_force = af;
+        s->steps = steps0;
+        s->cur = cur0;
+        if (rc != ISLBM_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(ISLBM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            s->graph = nullptr;
+            return fail(ISLBM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+        }
+        s->graph_steps = nsteps;
+    }
+    HIPCHK(hipGraphLaunch(s->graph, s->stream));
+    s->steps += nsteps;
+    return ISLBM_OK;
+}
+
+int islbm_stream(islbm_solver* s, void** stream)
+{
+    if (!s || !stream) return fail(ISLBM_ERR_ARG, "null argument");
+    *stream = (void*)s->stream;
+    return ISLBM_OK;
+}
+
+int islbm_device_bytes(islbm_solver* s, size_t* bytes)
+{
+    if (!s || !bytes) return fail(ISLBM_ERR_ARG, "null argument");
+    *bytes = s->bytes;
     return ISLBM_OK;
 }
 

@@ -1,0 +1,358 @@
+// islbm_serve.cpp -- continuous-batching serving loop over many resident solves (the
+// islbm_batch_* entry points of include/islbm.h).
+//
+// The reference runs one solve per program (its main, absent from the notes, loops
+// Launch_CollisionStreaming over one global state).  Here a request is one periodic-hill
+// solve with its own grid, Reynolds number, hill steepness and step count.  Requests are
+// admitted in submission order while the device-memory budget and the slot count allow;
+// an admitted solve keeps its populations resident, runs on its own stream and advances in
+// chunks replayed from a HIP graph (islbm_step_graph), two chunks in flight, so the solves
+// overlap on the device and a request that arrives while others run starts as soon as a
+// slot and memory free up.  A finished solve is retired (fields downloaded if requested,
+// a summary kept) and its memory returned to the budget.
+#include "../../include/islbm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_berr;
+
+int bfail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_berr = buf;
+    return code;
+}
+
+double now_s()
+{
+    using clk = std::chrono::steady_clock;
+    return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+constexpr int kDepth = 2;  // chunks in flight per solve
+
+struct Request {
+    int64_t id = 0;
+    islbm_params p{};
+    double u0[3] = {0, 0, 0};
+    double amp = 0;
+    uint64_t seed = 0;
+    int64_t nsteps = 0;
+    int64_t queued_steps = 0;  // steps enqueued on the device
+    int64_t done_steps = 0;    // steps whose chunk has completed
+    int keep = 0;
+    int state = ISLBM_REQ_QUEUED;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    size_t bytes = 0;
+    islbm_solver* s = nullptr;
+    hipEvent_t ev[kDepth] = {nullptr, nullptr};
+    int64_t ev_steps[kDepth] = {0, 0};  // steps covered by the chunk behind each event
+    int head = 0, inflight = 0;         // ring of in-flight chunk events
+    std::vector<double> fields;         // rho u v w after completion (keep != 0)
+    double summary[4] = {0, 0, 0, 0};   // nodes, mean rho, max |u|, mean v
+    std::string error;
+};
+
+}  // namespace
+
+struct islbm_batch {
+    int device = 0;
+    size_t budget = 0, used = 0;
+    int max_active = 1;
+    int chunk = 2;
+    int64_t next_id = 1;
+    std::deque<Request*> queue;
+    std::vector<Request*> active;
+    std::map<int64_t, Request*> all;
+    int64_t n_done = 0, n_failed = 0;
+};
+
+namespace {
+
+void release(islbm_batch* b, Request* r)
+{
+    for (hipEvent_t& e : r->ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (r->s) {
+        islbm_destroy(r->s);
+        r->s = nullptr;
+        b->used -= r->bytes;
+    }
+}
+
+void set_failed(islbm_batch* b, Request* r, const char* why)
+{
+    r->error = why ? why : "";
+    r->state = ISLBM_REQ_FAILED;
+    r->inflight = 0;
+    r->t_done = now_s();
+    release(b, r);
+    b->n_failed++;
+}
+
+// download the fields of a finished solve, keep them and/or a summary, free the solve
+int finish(islbm_batch* b, Request* r)
+{
+    const size_t n = (size_t)r->p.nx * r->p.ny * r->p.nz;
+    std::vector<double> f(4 * n);
+    int rc = islbm_download_macros(r->s, f.data(), f.data() + n, f.data() + 2 * n, f.data() + 3 * n);
+    if (rc) return rc;
+    double sr = 0, sv = 0, umax = 0;
+    for (size_t t = 0; t < n; ++t) {
+        sr += f[t];
+        sv += f[2 * n + t];
+        const double m = std::sqrt(f[n + t] * f[n + t] + f[2 * n + t] * f[2 * n + t] + f[3 * n + t] * f[3 * n + t]);
+        umax = std::isnan(m) ? NAN : std::max(umax, m);
+    }
+    r->summary[0] = (double)n;
+    r->summary[1] = sr / n;
+    r->summary[2] = umax;
+    r->summary[3] = sv / n;
+    if (r->keep) r->fields.swap(f);
+    r->state = ISLBM_REQ_DONE;
+    r->t_done = now_s();
+    release(b, r);
+    b->n_done++;
+    return ISLBM_OK;
+}
+
+// queue chunks of an active solve until kDepth are in flight or all steps are queued
+int feed(islbm_batch* b, Request* r)
+{
+    while (r->inflight < kDepth && r->queued_steps < r->nsteps) {
+        const int64_t left = r->nsteps - r->queued_steps;
+        const int n = (int)std::min<int64_t>(left, b->chunk);
+        int rc = (n == b->chunk) ? islbm_step_graph(r->s, n) : islbm_step(r->s, n);
+        if (rc) return rc;
+        void* st = nullptr;
+        islbm_stream(r->s, &st);
+        const int slot = (r->head + r->inflight) % kDepth;
+        if (hipEventRecord(r->ev[slot], (hipStream_t)st) != hipSuccess) return bfail(ISLBM_ERR_HIP, "event record");
+        r->ev_steps[slot] = n;
+        r->inflight++;
+        r->queued_steps += n;
+    }
+    return ISLBM_OK;
+}
+
+int admit(islbm_batch* b, Request* r)
+{
+    islbm_params p = r->p;
+    p.device = b->device;
+    int rc = islbm_create(&p, 0, &r->s);
+    if (rc) return rc;
+    b->used += r->bytes;
+    for (hipEvent_t& e : r->ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bfail(ISLBM_ERR_HIP, "event create");
+    if ((rc = islbm_init_equilibrium(r->s, 1.0, r->u0, r->amp, r->seed))) return rc;
+    r->state = ISLBM_REQ_ACTIVE;
+    r->t_admit = now_s();
+    return ISLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int islbm_batch_create(int32_t device, size_t hbm_budget, int32_t max_active, int32_t chunk, islbm_batch** out)
+{
+    if (!out || max_active <= 0 || chunk <= 0 || chunk % 2) return bfail(ISLBM_ERR_ARG, "bad batch arguments (chunk must be even)");
+    if (hipSetDevice(device) != hipSuccess) return bfail(ISLBM_ERR_HIP, "hipSetDevice(%d)", device);
+    islbm_batch* b = new islbm_batch();
+    b->device = device;
+    if (hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            delete b;
+            return bfail(ISLBM_ERR_HIP, "hipMemGetInfo");
+        }
+        hbm_budget = fr / 10 * 9;
+    }
+    b->budget = hbm_budget;
+    b->max_active = max_active;
+    b->chunk = chunk;
+    *out = b;
+    return ISLBM_OK;
+}
+
+int islbm_batch_destroy(islbm_batch* b)
+{
+    if (!b) return ISLBM_OK;
+    (void)hipSetDevice(b->device);
+    for (auto& kv : b->all) {
+        release(b, kv.second);
+        delete kv.second;
+    }
+    delete b;
+    return ISLBM_OK;
+}
+
+int islbm_batch_submit(islbm_batch* b, const islbm_params* p, const double u0[3], double amp, uint64_t seed,
+                       int64_t nsteps, int32_t keep_fields, int64_t* id)
+{
+    if (!b || !p || !u0 || !id || nsteps <= 0) return bfail(ISLBM_ERR_ARG, "bad submit arguments");
+    Request* r = new Request();
+    r->id = b->next_id++;
+    r->p = *p;
+    std::memcpy(r->u0, u0, sizeof r->u0);
+    r->amp = amp;
+    r->seed = seed;
+    r->nsteps = nsteps;
+    r->keep = keep_fields;
+    r->t_submit = now_s();
+    b->all[r->id] = r;
+    *id = r->id;
+    islbm_params q = *p;
+    q.device = b->device;
+    if (islbm_required_bytes(&q, &r->bytes) != ISLBM_OK) {
+        set_failed(b, r, "invalid parameters");
+        return ISLBM_OK;
+    }
+    if (r->bytes > b->budget) {
+        set_failed(b, r, "solve larger than the whole budget");
+        return ISLBM_OK;
+    }
+    b->queue.push_back(r);
+    return ISLBM_OK;
+}
+
+int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
+{
+    if (!b) return bfail(ISLBM_ERR_ARG, "null batch");
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(ISLBM_ERR_HIP, "hipSetDevice");
+    for (int pass = 0; pass < 2; ++pass) {
+        // retire completed chunks, in order per solve
+        bool progressed = false;
+        for (size_t t = 0; t < b->active.size();) {
+            Request* r = b->active[t];
+            while (r->state == ISLBM_REQ_ACTIVE && r->inflight > 0) {
+                const hipError_t q = hipEventQuery(r->ev[r->head]);
+                if (q == hipErrorNotReady) break;
+                if (q != hipSuccess) {
+                    set_failed(b, r, hipGetErrorString(q));
+                    break;
+                }
+                r->done_steps += r->ev_steps[r->head];
+                r->head = (r->head + 1) % kDepth;
+                r->inflight--;
+                progressed = true;
+            }
+            if (r->state == ISLBM_REQ_ACTIVE && r->done_steps == r->nsteps) {
+                int rc = finish(b, r);
+                if (rc) set_failed(b, r, islbm_last_error());
+            }
+            if (r->state != ISLBM_REQ_ACTIVE) {
+                b->active.erase(b->active.begin() + t);
+                progressed = true;
+            } else {
+                ++t;
+            }
+        }
+        // admit in submission order while a slot and memory are free
+        while (!b->queue.empty() && (int)b->active.size() < b->max_active &&
+               b->used + b->queue.front()->bytes <= b->budget) {
+            Request* r = b->queue.front();
+            b->queue.pop_front();
+            int rc = admit(b, r);
+            if (rc) {
+                set_failed(b, r, islbm_last_error());
+                continue;
+            }
+            b->active.push_back(r);
+            progressed = true;
+        }
+        // keep every active solve fed
+        for (Request* r : b->active) {
+            int rc = feed(b, r);
+            if (rc) set_failed(b, r, islbm_last_error());
+        }
+        if (!wait || progressed || b->active.empty()) break;
+        // block until the oldest in-flight chunk of the first active solve completes
+        for (Request* r : b->active)
+            if (r->inflight > 0) {
+                (void)hipEventSynchronize(r->ev[r->head]);
+                break;
+            }
+    }
+    if (out) {
+        out[0] = (int64_t)b->queue.size();
+        out[1] = (int64_t)b->active.size();
+        out[2] = b->n_done;
+        out[3] = b->n_failed;
+    }
+    return ISLBM_OK;
+}
+
+int islbm_batch_status(islbm_batch* b, int64_t id, int32_t* state, double times[3], int64_t* steps_done)
+{
+    if (!b) return bfail(ISLBM_ERR_ARG, "null batch");
+    auto it = b->all.find(id);
+    if (it == b->all.end()) return bfail(ISLBM_ERR_ARG, "unknown request %lld", (long long)id);
+    const Request* r = it->second;
+    if (state) *state = r->state;
+    if (times) {
+        times[0] = r->t_submit;
+        times[1] = r->t_admit;
+        times[2] = r->t_done;
+    }
+    if (steps_done) *steps_done = r->done_steps;
+    return ISLBM_OK;
+}
+
+int islbm_batch_result(islbm_batch* b, int64_t id, double* rho, double* u, double* v, double* w)
+{
+    if (!b) return bfail(ISLBM_ERR_ARG, "null batch");
+    auto it = b->all.find(id);
+    if (it == b->all.end()) return bfail(ISLBM_ERR_ARG, "unknown request %lld", (long long)id);
+    Request* r = it->second;
+    if (r->state != ISLBM_REQ_DONE) return bfail(ISLBM_ERR_STATE, "request %lld is not done", (long long)id);
+    if (r->fields.empty()) return bfail(ISLBM_ERR_STATE, "request %lld kept no fields", (long long)id);
+    const size_t n = (size_t)r->p.nx * r->p.ny * r->p.nz;
+    double* outs[4] = {rho, u, v, w};
+    for (int a = 0; a < 4; ++a)
+        if (outs[a]) std::memcpy(outs[a], r->fields.data() + a * n, n * sizeof(double));
+    std::vector<double>().swap(r->fields);
+    return ISLBM_OK;
+}
+
+int islbm_batch_summary(islbm_batch* b, int64_t id, double out[4])
+{
+    if (!b || !out) return bfail(ISLBM_ERR_ARG, "null argument");
+    auto it = b->all.find(id);
+    if (it == b->all.end()) return bfail(ISLBM_ERR_ARG, "unknown request %lld", (long long)id);
+    std::memcpy(out, it->second->summary, sizeof it->second->summary);
+    return ISLBM_OK;
+}
+
+int islbm_batch_bytes_in_use(islbm_batch* b, size_t* bytes)
+{
+    if (!b || !bytes) return bfail(ISLBM_ERR_ARG, "null argument");
+    *bytes = b->used;
+    return ISLBM_OK;
+}
+
+const char* islbm_batch_last_error(void) { return g_berr.c_str(); }
+
+double islbm_batch_now(void) { return now_s(); }
+
+}  // extern "C"

@@ -30,7 +30,12 @@ HEADER_SYMBOLS = [
     "islbm_selftest_collision", "islbm_copy_bandwidth",
     "islbm_ref_init", "islbm_ref_shutdown", "islbm_ref_device_arrays",
     "Launch_CollisionStreaming", "Launch_ModifyForcingTerm",
+    "islbm_step_graph", "islbm_stream", "islbm_device_bytes",
+    "islbm_batch_create", "islbm_batch_destroy", "islbm_batch_submit", "islbm_batch_advance",
+    "islbm_batch_status", "islbm_batch_result", "islbm_batch_summary", "islbm_batch_bytes_in_use",
+    "islbm_batch_last_error", "islbm_batch_now",
 ]
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
 class Params(ctypes.Structure):
@@ -95,6 +100,20 @@ def lib():
     L.Launch_CollisionStreaming.restype = None
     L.Launch_ModifyForcingTerm.argtypes = []
     L.Launch_ModifyForcingTerm.restype = None
+    L.islbm_step_graph.argtypes = [vp, i32]
+    L.islbm_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.islbm_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    i64, i64p = ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)
+    L.islbm_batch_create.argtypes = [i32, ctypes.c_size_t, i32, i32, ctypes.POINTER(vp)]
+    L.islbm_batch_destroy.argtypes = [vp]
+    L.islbm_batch_submit.argtypes = [vp, pp, dp, ctypes.c_double, ctypes.c_uint64, i64, i32, i64p]
+    L.islbm_batch_advance.argtypes = [vp, i32, i64p]
+    L.islbm_batch_status.argtypes = [vp, i64, ctypes.POINTER(i32), dp, i64p]
+    L.islbm_batch_result.argtypes = [vp, i64, dp, dp, dp, dp]
+    L.islbm_batch_summary.argtypes = [vp, i64, dp]
+    L.islbm_batch_bytes_in_use.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.islbm_batch_last_error.restype = ctypes.c_char_p
+    L.islbm_batch_now.restype = ctypes.c_double
     for name in HEADER_SYMBOLS:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int and name not in ("islbm_last_error",):
@@ -186,6 +205,17 @@ class Solver:
         if sync:
             self.sync()
 
+    def step_graph(self, n: int, sync: bool = True):
+        """n steps (even) replayed from a HIP graph recorded on the first call of this n."""
+        _check(lib().islbm_step_graph(self.h, int(n)), "islbm_step_graph")
+        if sync:
+            self.sync()
+
+    def device_bytes(self) -> int:
+        v = ctypes.c_size_t()
+        _check(lib().islbm_device_bytes(self.h, ctypes.byref(v)), "islbm_device_bytes")
+        return v.value
+
     def sync(self):
         _check(lib().islbm_sync(self.h), "islbm_sync")
 
@@ -241,6 +271,93 @@ class Solver:
         o = np.zeros(2)
         _check(lib().islbm_time_steps(self.h, int(n), _dp(o)), "islbm_time_steps")
         return float(o[0]), float(o[1])
+
+
+def _bcheck(rc: int, what: str) -> int:
+    if rc < 0:
+        L = lib()
+        msg = L.islbm_batch_last_error().decode(errors="replace")
+        inner = L.islbm_last_error().decode(errors="replace")
+        raise ISLBMError(f"{what} failed ({rc}): {msg} {inner}".strip())
+    return rc
+
+
+class Batch:
+    """Continuous-batching serving loop over many device-resident solves (islbm_batch_*)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_active: int = 8, chunk: int = 50):
+        h = ctypes.c_void_p()
+        _bcheck(lib().islbm_batch_create(int(device), int(hbm_budget), int(max_active), int(chunk), ctypes.byref(h)),
+                "islbm_batch_create")
+        self.h = h
+        self.params = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().islbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def now() -> float:
+        return float(lib().islbm_batch_now())
+
+    def submit(self, params: Params, nsteps: int, u0=(0.0, 0.0, 0.0), amp: float = 0.0, seed: int = 20261203,
+               keep_fields: bool = False) -> int:
+        u = np.ascontiguousarray(u0, dtype=np.float64)
+        rid = ctypes.c_int64()
+        _bcheck(lib().islbm_batch_submit(self.h, ctypes.byref(params), _dp(u), float(amp), int(seed), int(nsteps),
+                                         int(bool(keep_fields)), ctypes.byref(rid)), "islbm_batch_submit")
+        self.params[rid.value] = (params.nz, params.ny, params.nx)
+        return rid.value
+
+    def advance(self, wait: bool = True):
+        """One pass of the loop; returns (queued, active, done, failed)."""
+        c = (ctypes.c_int64 * 4)()
+        _bcheck(lib().islbm_batch_advance(self.h, int(bool(wait)), c), "islbm_batch_advance")
+        return tuple(int(x) for x in c)
+
+    def run(self):
+        """Advance until nothing is queued or active."""
+        while True:
+            q, a, d, f = self.advance(True)
+            if q == 0 and a == 0:
+                return d, f
+
+    def status(self, rid: int):
+        st = ctypes.c_int32()
+        t = np.zeros(3)
+        steps = ctypes.c_int64()
+        _bcheck(lib().islbm_batch_status(self.h, int(rid), ctypes.byref(st), _dp(t), ctypes.byref(steps)),
+                "islbm_batch_status")
+        return {"state": st.value, "t_submit": t[0], "t_admit": t[1], "t_done": t[2], "steps": steps.value}
+
+    def result(self, rid: int):
+        shape = self.params[rid]
+        r, u, v, w = (np.empty(shape) for _ in range(4))
+        _bcheck(lib().islbm_batch_result(self.h, int(rid), _dp(r), _dp(u), _dp(v), _dp(w)), "islbm_batch_result")
+        return r, u, v, w
+
+    def summary(self, rid: int) -> dict:
+        o = np.zeros(4)
+        _bcheck(lib().islbm_batch_summary(self.h, int(rid), _dp(o)), "islbm_batch_summary")
+        return dict(zip(["nodes", "mean_rho", "max_u", "mean_v"], o.tolist()))
+
+    def bytes_in_use(self) -> int:
+        v = ctypes.c_size_t()
+        _bcheck(lib().islbm_batch_bytes_in_use(self.h, ctypes.byref(v)), "islbm_batch_bytes_in_use")
+        return v.value
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20, device: int = 0) -> float:

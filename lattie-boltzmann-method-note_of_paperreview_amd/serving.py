@@ -1,0 +1,98 @@
+"""Serving workload: a stream of periodic-hill solve requests with Poisson arrivals, run
+through the continuous-batching loop (islbm_batch_*), and the one-solve-at-a-time
+comparator.
+
+Request draws (reference axes: x spanwise, y streamwise, z wall-normal): streamwise nodes
+from {64, 96, 128, 192, 256}, spanwise = streamwise / 2, wall-normal from {64, 96, 128};
+Re log-uniform in [100, 5600]; hill steepness alpha uniform in [0.5, 3.0] with the hill
+length scaled by alpha (LY = 5.142 + 3.858 alpha keeps the flat section of the reference
+hill, model.h); U_ref 0.05; FP32 storage; Smagorinsky Cs 0.1.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import islbm
+
+
+def workload(n: int, seed: int = 20261203, nsteps: int = 5000):
+    """n request dicts (params keywords, step count, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        ny = int(rng.choice([64, 96, 128, 192, 256]))
+        nz = int(rng.choice([64, 96, 128]))
+        re = float(np.exp(rng.uniform(np.log(100.0), np.log(5600.0))))
+        alpha = float(rng.uniform(0.5, 3.0))
+        kw = dict(nx=ny // 2, ny=ny, nz=nz, re=re, tau=0.0, u_ref=0.05, cs_smag=0.1, hill_alpha=alpha,
+                  ly=5.142 + 3.858 * alpha, mesh_mode=1, stretch=2.0)
+        out.append(dict(kw=kw, nsteps=nsteps, seed=seed + k))
+    return out
+
+
+def offered_rate(reqs, glups: float) -> float:
+    """Arrival rate (requests / s) at which the offered work is 1.5x a service rate of
+    `glups` 1e9 lattice updates per second."""
+    work = np.mean([r["kw"]["nx"] * r["kw"]["ny"] * r["kw"]["nz"] * r["nsteps"] for r in reqs])
+    return 1.5 * glups * 1e9 / work
+
+
+def run_batched(reqs, rate: float, seed: int = 7, precision: str = "fp32", max_active: int = 8, chunk: int = 50,
+                hbm_budget: int = 0):
+    """Submit each request at its Poisson arrival time (wall clock, from the start) and run
+    the loop until all finish.  Returns latencies (s), aggregate MLUPS and counts."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs))) if rate > 0 else np.zeros(len(reqs))
+    arrivals -= arrivals[0]
+    ids = []
+    with islbm.Batch(max_active=max_active, chunk=chunk, hbm_budget=hbm_budget) as b:
+        t0 = b.now()
+        nxt = 0
+        active = 0
+        while True:
+            now = b.now() - t0
+            while nxt < len(reqs) and arrivals[nxt] <= now:
+                r = reqs[nxt]
+                p = islbm.default_params(**r["kw"], precision=precision)
+                ids.append((b.submit(p, r["nsteps"], u0=(0.0, 0.05, 0.0), amp=1e-3, seed=r["seed"]), r))
+                nxt += 1
+            # block for a chunk only while solves run (arrivals are polled between chunks)
+            q, active, d, f = b.advance(wait=active > 0)
+            if nxt == len(reqs) and q == 0 and active == 0:
+                break
+            if q == 0 and active == 0:
+                time.sleep(max(0.0, min(arrivals[nxt] - (b.now() - t0), 0.01)))
+        t_end = b.now() - t0
+        lat, lups, finite, failed = [], 0.0, 0, 0
+        for rid, r in ids:
+            st = b.status(rid)
+            if st["state"] != islbm.REQ_DONE:
+                failed += 1
+                continue
+            lat.append(st["t_done"] - st["t_submit"])
+            kw = r["kw"]
+            lups += kw["nx"] * kw["ny"] * kw["nz"] * r["nsteps"]
+            finite += int(np.isfinite(b.summary(rid)["max_u"]))
+    lat = np.array(lat)
+    return {"requests": len(reqs), "done": int(lat.size), "failed": failed, "finite_results": finite,
+            "p50_s": float(np.percentile(lat, 50)) if lat.size else None,
+            "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
+            "wall_s": float(t_end), "aggregate_mlups": float(lups / t_end / 1e6) if t_end > 0 else None}
+
+
+def run_sequential(reqs, precision: str = "fp64", kernel: int = islbm.KERNEL_PERNODE):
+    """One solve at a time on the GPU (create, init, step, download), for the comparison."""
+    t0 = time.perf_counter()
+    lups = 0.0
+    for r in reqs:
+        p = islbm.default_params(**r["kw"], precision=precision, kernel=kernel)
+        with islbm.Solver(p) as s:
+            s.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=r["seed"])
+            s.step(r["nsteps"])
+            s.macros()
+        kw = r["kw"]
+        lups += kw["nx"] * kw["ny"] * kw["nz"] * r["nsteps"]
+    dt = time.perf_counter() - t0
+    return {"requests": len(reqs), "wall_s": dt, "aggregate_mlups": lups / dt / 1e6}

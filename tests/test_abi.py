@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "islbm.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:int|void|const char\*)\s+\**\s*([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    names = re.findall(r"^\s*(?:int|void|double|const char\*)\s+\**\s*([A-Za-z_]\w*)\s*\(", text, flags=re.M)
     return sorted(set(names))
 
 
@@ -79,3 +79,11 @@ def test_collision_forms_agree_on_host(pkg):
     assert pkg.lib().islbm_selftest_collision(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
     assert out[0] < 1e-15 and out[1] < 1e-15, out
     assert out[2] < 1e-7 and out[3] < 5e-7, out
+
+
+@pytest.mark.parametrize("kw", [dict(chunk=3), dict(chunk=0), dict(max_active=0)])
+def test_batch_rejects_bad_arguments_before_touching_the_gpu(pkg, kw):
+    args = dict(device=0, hbm_budget=1 << 30, max_active=4, chunk=10)
+    args.update(kw)
+    with pytest.raises(pkg.ISLBMError):
+        pkg.Batch(**args)

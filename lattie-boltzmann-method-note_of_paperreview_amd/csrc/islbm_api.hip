@@ -274,20 +274,6 @@ int get_f(islbm_solver* s, int buf, double* f)
     return ISLBM_OK;
 }
 
-int put_macros(islbm_solver* s, const double* const* m4)
-{
-    const long long n = s->n;
-    for (int a = 0; a < 4; ++a) {
-        if (s->fp64) {
-            HIPCHK(hipMemcpy(s->mac[a], m4[a], n * 8, hipMemcpyHostToDevice));
-        } else {
-            std::vector<float> h(m4[a], m4[a] + n);
-            HIPCHK(hipMemcpy(s->mac[a], h.data(), n * 4, hipMemcpyHostToDevice));
-        }
-    }
-    return ISLBM_OK;
-}
-
 StepLaunch make_launch(islbm_solver* s)
 {
     StepLaunch L{};
@@ -441,44 +427,13 @@ int islbm_init_equilibrium(islbm_solver* s, double rho0, const double u0[3], dou
 {
     if (!s || !u0) return fail(ISLBM_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(s->p.device));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
-    const long long n = s->n;
-    std::vector<double> f((size_t)NQ * n), m((size_t)4 * n);
-    for (int j = 0; j < ny; ++j)
-        for (int k = 0; k < nz; ++k)
-            for (int i = 0; i < nx; ++i) {
-                const long long id = ((long long)k * ny + j) * nx + i;
-                const uint64_t base = (((uint64_t)j * nz + k) * nx + i) * 3;
-                double uu[3];
-                for (int c = 0; c < 3; ++c) {
-                    double d = 0.0;
-                    if (amp != 0.0) {
-                        // SplitMix64 finaliser on the counter
-                        uint64_t z = seed + (base + c + 1) * 0x9E3779B97F4A7C15ULL;
-                        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-                        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-                        z = z ^ (z >> 31);
-                        d = amp * (2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
-                    }
-                    uu[c] = u0[c] + d;
-                }
-                const double udot = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
-                // initialization.h:36-41
-                f[id] = wq(0) * rho0 * (1.0 - 1.5 * udot);
-                for (int q = 1; q < NQ; ++q) {
-                    const double eu = cx(q) * uu[0] + cy(q) * uu[1] + cz(q) * uu[2];
-                    f[(size_t)q * n + id] = wq(q) * rho0 * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * udot);
-                }
-                m[id] = rho0; m[n + id] = uu[0]; m[2 * n + id] = uu[1]; m[3 * n + id] = uu[2];
-            }
+    // computed on the device (k_init_equilibrium): no host arrays, no upload
     s->cur = 0;
-    int rc = put_f(s, 0, f.data());
-    if (rc) return rc;
-    const double* m4[4] = {m.data(), m.data() + n, m.data() + 2 * n, m.data() + 3 * n};
-    if ((rc = put_macros(s, m4))) return rc;
+    HIPCHK(launch_init_equilibrium(s->stream, s->store, s->f[0], s->mac[0], s->n, s->p.nx, s->p.ny, s->p.nz, rho0, u0,
+                                   amp, seed, s->scale));
     const double sc[3] = {s->g.force0, 0.0, 0.0};
-    HIPCHK(hipMemcpy(s->scal, sc, sizeof sc, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(s->scal, sc, sizeof sc, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     s->steps = 0;
     return ISLBM_OK;
 }

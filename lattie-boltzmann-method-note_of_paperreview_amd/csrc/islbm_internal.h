@@ -78,6 +78,13 @@ struct StepLaunch {
 };
 hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L);
 hipError_t launch_stream_copy(hipStream_t st, const void* src, void* dst, long long bytes);
+struct InitScale {
+    float s[NQ];
+};
+// f = f_eq(rho0, u0 + seeded delta) in the population format and rho, u, v, w (the four
+// consecutive arrays at mac) in the compute precision; scale = 16-bit scale per direction.
+hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, void* mac, long long n, int nx, int ny, int nz,
+                                   double rho0, const double u0[3], double amp, uint64_t seed, const float* scale);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
 
 hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* part,

@@ -275,6 +275,92 @@ hipError_t launch_unique_to_padded(hipStream_t st, const PtrSet& uni, const PtrS
 }
 
 // ------------------------------------------------------------------------------------
+// Seeded equilibrium start on the device (InitialUsingDftFunc, initialization.h:6-49, with
+// the seeded velocity perturbation): f = f_eq(rho0, u0 + delta) in FP64, stored in the
+// population format; delta_c ~ U(-amp, amp) from the SplitMix64 finaliser of the counter
+// ((j nz + k) nx + i) 3 + c.
+// ------------------------------------------------------------------------------------
+__device__ inline uint64_t splitmix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+template <Store ST>
+__global__ __launch_bounds__(256) void k_init_equilibrium(void* f, void* mac, long long n, int nx, int ny, int nz,
+                                                          double rho0, double u0x, double u0y, double u0z, double amp,
+                                                          uint64_t seed, InitScale sc)
+{
+    using CD = Codec<ST>;
+    using raw = typename CD::raw;
+    using C = typename CD::C;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((long long)nx * ny));
+    const uint64_t base = (((uint64_t)j * nz + k) * nx + i) * 3;
+    const double u0[3] = {u0x, u0y, u0z};
+    double uu[3];
+    for (int c = 0; c < 3; ++c) {
+        double d = 0.0;
+        if (amp != 0.0) {
+            const uint64_t z = splitmix64(seed + (base + c + 1) * 0x9E3779B97F4A7C15ULL);
+            d = amp * (2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
+        }
+        uu[c] = u0[c] + d;
+    }
+    const double udot = uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2];
+    raw* F = (raw*)f;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double eu = cx(q) * uu[0] + cy(q) * uu[1] + cz(q) * uu[2];
+        const double fq = q == 0 ? wq(0) * rho0 * (1.0 - 1.5 * udot)
+                                 : wq(q) * rho0 * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * udot);
+        if constexpr (ST == Store::F64) {
+            F[(long long)q * n + id] = fq;
+        } else if constexpr (ST == Store::F32) {
+            F[(long long)q * n + id] = (float)(fq - wq(q));
+        } else {
+            // scaled deviation rounded to FP32 first, then to the 16-bit format
+            const float v = (float)((fq - wq(q)) * (double)sc.s[q]);
+            CD::st(F, (long long)q * n + id, v, 1.0f);
+        }
+    }
+    C* M = (C*)mac;
+    M[id] = (C)rho0;
+    M[n + id] = (C)uu[0];
+    M[2 * n + id] = (C)uu[1];
+    M[3 * n + id] = (C)uu[2];
+}
+
+hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, void* mac, long long n, int nx, int ny, int nz,
+                                   double rho0, const double u0[3], double amp, uint64_t seed, const float* scale)
+{
+    InitScale sc;
+    for (int q = 0; q < NQ; ++q) sc.s[q] = scale ? scale[q] : 1.0f;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    switch (store) {
+    case Store::F64:
+        hipLaunchKernelGGL(k_init_equilibrium<Store::F64>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0, u0[0],
+                           u0[1], u0[2], amp, seed, sc);
+        break;
+    case Store::F32:
+        hipLaunchKernelGGL(k_init_equilibrium<Store::F32>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0, u0[0],
+                           u0[1], u0[2], amp, seed, sc);
+        break;
+    case Store::BF16:
+        hipLaunchKernelGGL(k_init_equilibrium<Store::BF16>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0,
+                           u0[0], u0[1], u0[2], amp, seed, sc);
+        break;
+    case Store::FP16:
+        hipLaunchKernelGGL(k_init_equilibrium<Store::FP16>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0,
+                           u0[0], u0[1], u0[2], amp, seed, sc);
+        break;
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
 // Plain streaming copy (float4), the bandwidth reference and counter calibration kernel:
 // one float4 per thread, as many workgroups as the copy needs (a grid-stride loop over a
 // fixed grid reaches about three quarters of this rate on MI355X)

@@ -123,6 +123,56 @@ int islbm_set_rho_modify(islbm_solver* s, double drho);
 /* Current value of the bulk-flux accumulator (sum of v*dx*dz over the probe plane). */
 int islbm_get_flux_sum(islbm_solver* s, double* sum);
 
+/* Host -> device macroscopic fields ([nz][ny][nx] doubles; NULL skips a field): the state a
+ * restart needs beside the populations (the flux probe reads v of the previous step). */
+int islbm_upload_macros(islbm_solver* s, const double* rho, const double* u, const double* v, const double* w);
+/* The parameters the solver was created with. */
+int islbm_get_params(islbm_solver* s, islbm_params* p);
+
+/* Global mass correction: sets rho_modify (added to F0 before the moments of the next step,
+ * evolution.h:597) to rho0 - mean(rho) of the last step, with the mean from a deterministic
+ * FP64 device reduction.  The reference's computation of rho_modify lives in its absent
+ * main; this restores the mean density to rho0 in one step. */
+int islbm_mass_correct(islbm_solver* s, double rho0, double* mean_rho);
+/* Launch_Monitor (monitor.h:4-26): out = [v, u, w, force] at the reference's monitor node
+ * (NX6/2, NYD6/2, NZ6/2 of the padded single-rank layout). */
+int islbm_monitor(islbm_solver* s, double out[4]);
+
+/* Turbulence statistics (Launch_TurbulentSum, statistics.h:122-249): FP64 sums over samples of
+ * U V W P UU UV UW VV VW WW PU PV PW PP KT, the nine squared velocity derivatives
+ * DUDX2 .. DWDZ2 and UUU UUV UUW VVU VVV VVW WWU WWV WWW, [nz][ny][nx] each.
+ * enable allocates and zeroes them (33 * 8 * nodes bytes beside islbm_required_bytes);
+ * every > 0 takes a sample after every `every`-th step of islbm_step, every = 0 only on
+ * islbm_stats_accumulate.  d/dx, d/dy: the reference's 4th-order central differences
+ * (statistics.h:188-194); d/dz: 5-point derivative weights on the node's vertical mesh line
+ * over the reference's window (statistics.h:196-199). */
+int islbm_stats_enable(islbm_solver* s, int32_t every);
+int islbm_stats_disable(islbm_solver* s);
+int islbm_stats_accumulate(islbm_solver* s);
+int islbm_stats_reset(islbm_solver* s);
+int islbm_stats_count(islbm_solver* s, int64_t* count);
+int islbm_stats_set_count(islbm_solver* s, int64_t count);
+/* names[0..32] = the accumulator names, in storage order; returns 33. */
+int islbm_stats_names(const char** names, int32_t cap);
+int islbm_stats_download(islbm_solver* s, const char* name, double* out);
+int islbm_stats_upload(islbm_solver* s, const char* name, const double* in);
+
+/* The reference's on-disk formats (fileIO.h), for interoperability with its dumps.
+ * Padded single-rank arrays of (NX+7)*(NY+7)*(NZ+6) doubles, index J*NX6*NZ6 + K*NX6 + I.
+ * checkpoint_write: dir/result/{rho,u,v,w,f0..f18}_<rank>.bin and, for rank 0,
+ *   dir/result/0_force.dat (OutputData, fileIO_PDF, fileIO.h:52-67, 262-292).
+ * checkpoint_read: the same files back into the solver (InitialUsingBkpData, fileIO.h:314-354).
+ * stats_write: dir/statistics/NAME/NAME_<rank>.bin over the interior with its duplicate
+ *   planes, k outer, i inner, and dir/statistics/accu.dat (Launch_OutputTB, fileIO.h:379-451).
+ * stats_read: InitialTBUsingBkpData (fileIO.h:453-508).
+ * monitor_append: one "step v u w force" line of Monitor.dat (monitor.h:4-26). */
+int islbm_checkpoint_write(islbm_solver* s, const char* dir, int32_t rank);
+int islbm_checkpoint_read(islbm_solver* s, const char* dir, int32_t rank);
+int islbm_stats_write(islbm_solver* s, const char* dir, int32_t rank);
+int islbm_stats_read(islbm_solver* s, const char* dir, int32_t rank);
+int islbm_monitor_append(islbm_solver* s, const char* path);
+const char* islbm_io_last_error(void);
+
 /* Derived scalars: [dt, tau, nu, uref, a, dx, dy, force0]. */
 int islbm_scalars(islbm_solver* s, double out[8]);
 /* Steps taken since creation / the last init. */

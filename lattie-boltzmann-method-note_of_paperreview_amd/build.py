@@ -20,8 +20,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libislbm_amd.so")
 
-SOURCES = ["islbm_geometry.cpp", "islbm_kernels.hip", "islbm_tiled.hip", "islbm_api.hip",
-           "islbm_serve.cpp"]
+SOURCES = ["islbm_geometry.cpp", "islbm_kernels.hip", "islbm_tiled.hip", "islbm_api.hip", "islbm_stats.hip",
+           "islbm_serve.cpp", "islbm_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

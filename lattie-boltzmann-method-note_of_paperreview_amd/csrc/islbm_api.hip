@@ -83,6 +83,7 @@ size_t table_bytes(const islbm_params& p)
     b += nz * ny * cb;                               // LES filter width
     b += nz * nx * 8 + nz * 8;                       // flux weights, per-level flux partials
     b += 8 * 8;                                      // scalars
+    b += (kSumBlocks + 1) * 8;                       // reduction partials
     return b;
 }
 
@@ -126,6 +127,12 @@ struct islbm_solver {
     std::vector<hipEvent_t> evpool;
     hipGraphExec_t graph = nullptr;  // islbm_step_graph: the captured chunk
     int graph_steps = 0;
+    double* red = nullptr;           // kSumBlocks partials + the sum (mass correction)
+    // turbulence statistics (islbm_stats_enable)
+    double* st_acc = nullptr;        // [NSTAT][n] FP64 sums
+    double* st_zdw = nullptr;        // [ny][nz][5] wall-normal derivative weights
+    int st_every = 0;
+    long long st_count = 0;
 };
 
 namespace {
@@ -138,6 +145,8 @@ void free_all(islbm_solver* s)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    for (void* q : {(void*)s->red, (void*)s->st_acc, (void*)s->st_zdw})
+        if (q) (void)hipFree(q);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -316,6 +325,11 @@ int enqueue_step(islbm_solver* s, hipEvent_t ev0, hipEvent_t ev1)
     if (ev1) HIPCHK(hipEventRecord(ev1, s->stream));
     s->cur = 1 - s->cur;
     s->steps++;
+    if (s->st_acc && s->st_every > 0 && s->steps % s->st_every == 0) {
+        HIPCHK(launch_stats(s->stream, s->fp64, s->mac[0], s->st_acc, s->st_zdw, s->g.dx, s->g.dy, s->p.nx, s->p.ny,
+                            s->p.nz));
+        s->st_count++;
+    }
     if (s->auto_force && s->steps % s->p.ndtfrc == 0) return enqueue_force_update(s);
     return ISLBM_OK;
 }
@@ -405,6 +419,8 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
     if (hipMemset(s->mac[0], 0, 4 * mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset macros"));
     for (int a = 1; a < 4; ++a) s->mac[a] = (char*)s->mac[0] + a * mb;
     if (hipMalloc((void**)&s->scal, 8 * sizeof(double)) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc scalars"));
+    if (hipMalloc((void**)&s->red, (kSumBlocks + 1) * sizeof(double)) != hipSuccess)
+        return bail(fail(ISLBM_ERR_HIP, "hipMalloc reduction buffer"));
     const double sc[8] = {s->g.force0, 0.0, 0.0, 0, 0, 0, 0, 0};
     if (hipMemcpy(s->scal, sc, sizeof sc, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "scalars"));
     int rc = s->fp64 ? upload_tables<double>(s) : upload_tables<float>(s);
@@ -488,8 +504,10 @@ int islbm_step_graph(islbm_solver* s, int32_t nsteps)
 {
     if (!s || nsteps <= 0 || nsteps % 2) return fail(ISLBM_ERR_ARG, "islbm_step_graph: nsteps must be even and positive");
     HIPCHK(hipSetDevice(s->p.device));
-    // a forcing update inside the chunk is tied to the absolute step count: step eagerly
+    // a forcing update inside the chunk is tied to the absolute step count, and so is the
+    // statistics cadence: step eagerly
     if (s->auto_force && (s->steps + nsteps) / s->p.ndtfrc > s->steps / s->p.ndtfrc) return islbm_step(s, nsteps);
+    if (s->st_acc && s->st_every > 0) return islbm_step(s, nsteps);
     if (!s->graph || s->graph_steps != nsteps) {
         if (s->graph) {
             HIPCHK(hipGraphExecDestroy(s->graph));
@@ -539,6 +557,171 @@ int islbm_device_bytes(islbm_solver* s, size_t* bytes)
 {
     if (!s || !bytes) return fail(ISLBM_ERR_ARG, "null argument");
     *bytes = s->bytes;
+    return ISLBM_OK;
+}
+
+int islbm_upload_macros(islbm_solver* s, const double* rho, const double* u, const double* v, const double* w)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const double* in[4] = {rho, u, v, w};
+    std::vector<float> h;
+    for (int a = 0; a < 4; ++a) {
+        if (!in[a]) continue;
+        if (s->fp64) {
+            HIPCHK(hipMemcpy(s->mac[a], in[a], s->n * 8, hipMemcpyHostToDevice));
+        } else {
+            h.assign(in[a], in[a] + s->n);
+            HIPCHK(hipMemcpy(s->mac[a], h.data(), s->n * 4, hipMemcpyHostToDevice));
+        }
+    }
+    return ISLBM_OK;
+}
+
+int islbm_mass_correct(islbm_solver* s, double rho0, double* mean_rho)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(launch_sum(s->stream, s->fp64, s->mac[0], s->n, s->red, s->red + kSumBlocks));
+    double sum = 0.0;
+    HIPCHK(hipMemcpyAsync(&sum, s->red + kSumBlocks, sizeof sum, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const double mean = sum / (double)s->n;
+    const double dr = rho0 - mean;
+    HIPCHK(hipMemcpy(s->scal + 1, &dr, sizeof dr, hipMemcpyHostToDevice));
+    if (mean_rho) *mean_rho = mean;
+    return ISLBM_OK;
+}
+
+int islbm_monitor(islbm_solver* s, double out[4])
+{
+    if (!s || !out) return fail(ISLBM_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // the reference's monitor node (NX6/2, NYD6/2, NZ6/2) of the padded single-rank layout
+    const int i = (s->p.nx + 7) / 2 - 3, j = (s->p.ny + 7) / 2 - 3, k = (s->p.nz + 6) / 2 - 3;
+    const long long id = ((long long)k * s->p.ny + j) * s->p.nx + i;
+    const int order[3] = {2, 1, 3};  // v, u, w (monitor.h:19)
+    for (int t = 0; t < 3; ++t) {
+        if (s->fp64) {
+            HIPCHK(hipMemcpy(&out[t], (const double*)s->mac[order[t]] + id, 8, hipMemcpyDeviceToHost));
+        } else {
+            float f = 0.f;
+            HIPCHK(hipMemcpy(&f, (const float*)s->mac[order[t]] + id, 4, hipMemcpyDeviceToHost));
+            out[t] = f;
+        }
+    }
+    HIPCHK(hipMemcpy(&out[3], s->scal + 0, 8, hipMemcpyDeviceToHost));
+    return ISLBM_OK;
+}
+
+int islbm_stats_enable(islbm_solver* s, int32_t every)
+{
+    if (!s || every < 0) return fail(ISLBM_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const size_t nacc = (size_t)NSTAT * s->n;
+    if (!s->st_acc) {
+        HIPCHK(hipMalloc((void**)&s->st_acc, nacc * sizeof(double)));
+        const std::vector<double> zw = zderiv_weights(s->g);
+        HIPCHK(hipMalloc((void**)&s->st_zdw, zw.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(s->st_zdw, zw.data(), zw.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemset(s->st_acc, 0, nacc * sizeof(double)));
+    s->st_every = every;
+    s->st_count = 0;
+    return ISLBM_OK;
+}
+
+int islbm_stats_disable(islbm_solver* s)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (double** q : {&s->st_acc, &s->st_zdw})
+        if (*q) {
+            HIPCHK(hipFree(*q));
+            *q = nullptr;
+        }
+    s->st_every = 0;
+    s->st_count = 0;
+    return ISLBM_OK;
+}
+
+int islbm_stats_accumulate(islbm_solver* s)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    if (!s->st_acc) return fail(ISLBM_ERR_STATE, "statistics are not enabled");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(launch_stats(s->stream, s->fp64, s->mac[0], s->st_acc, s->st_zdw, s->g.dx, s->g.dy, s->p.nx, s->p.ny,
+                        s->p.nz));
+    s->st_count++;
+    return ISLBM_OK;
+}
+
+int islbm_stats_reset(islbm_solver* s)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    if (!s->st_acc) return fail(ISLBM_ERR_STATE, "statistics are not enabled");
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemset(s->st_acc, 0, (size_t)NSTAT * s->n * sizeof(double)));
+    s->st_count = 0;
+    return ISLBM_OK;
+}
+
+int islbm_stats_count(islbm_solver* s, int64_t* count)
+{
+    if (!s || !count) return fail(ISLBM_ERR_ARG, "null argument");
+    *count = s->st_count;
+    return ISLBM_OK;
+}
+
+int islbm_stats_set_count(islbm_solver* s, int64_t count)
+{
+    if (!s || count < 0) return fail(ISLBM_ERR_ARG, "bad argument");
+    s->st_count = count;
+    return ISLBM_OK;
+}
+
+namespace {
+int stat_index(const char* name)
+{
+    for (int a = 0; a < NSTAT; ++a)
+        if (std::strcmp(name, kStatNames[a]) == 0) return a;
+    return -1;
+}
+}  // namespace
+
+int islbm_stats_names(const char** names, int32_t cap)
+{
+    if (!names || cap < NSTAT) return -NSTAT;
+    for (int a = 0; a < NSTAT; ++a) names[a] = kStatNames[a];
+    return NSTAT;
+}
+
+int islbm_stats_download(islbm_solver* s, const char* name, double* out)
+{
+    if (!s || !name || !out) return fail(ISLBM_ERR_ARG, "null argument");
+    if (!s->st_acc) return fail(ISLBM_ERR_STATE, "statistics are not enabled");
+    const int a = stat_index(name);
+    if (a < 0) return fail(ISLBM_ERR_ARG, "unknown statistic %s", name);
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(out, s->st_acc + (size_t)a * s->n, s->n * sizeof(double), hipMemcpyDeviceToHost));
+    return ISLBM_OK;
+}
+
+int islbm_stats_upload(islbm_solver* s, const char* name, const double* in)
+{
+    if (!s || !name || !in) return fail(ISLBM_ERR_ARG, "null argument");
+    if (!s->st_acc) return fail(ISLBM_ERR_STATE, "statistics are not enabled");
+    const int a = stat_index(name);
+    if (a < 0) return fail(ISLBM_ERR_ARG, "unknown statistic %s", name);
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(s->st_acc + (size_t)a * s->n, in, s->n * sizeof(double), hipMemcpyHostToDevice));
     return ISLBM_OK;
 }
 
@@ -599,6 +782,13 @@ int islbm_scalars(islbm_solver* s, double out[8])
     const HostGeom& g = s->g;
     const double v[8] = {g.dt, g.tau, g.nu, g.uref, g.a, g.dx, g.dy, g.force0};
     std::memcpy(out, v, sizeof v);
+    return ISLBM_OK;
+}
+
+int islbm_get_params(islbm_solver* s, islbm_params* p)
+{
+    if (!s || !p) return fail(ISLBM_ERR_ARG, "null argument");
+    *p = s->p;
     return ISLBM_OK;
 }
 

@@ -248,4 +248,30 @@ std::string build_host_geometry(const islbm_params& p, HostGeom& g)
     return "";
 }
 
+std::vector<double> zderiv_weights(const HostGeom& g)
+{
+    const int ny = g.ny, nz = g.nz;
+    std::vector<double> w((size_t)ny * nz * 5, 0.0);
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+            int n = k - 2;
+            n = n < 0 ? 0 : (n > nz - 5 ? nz - 5 : n);
+            const double* zl = &g.zc[(size_t)j * nz + n];
+            const double x = g.zc[(size_t)j * nz + k];
+            for (int m = 0; m < 5; ++m) {
+                // L_m'(x) = sum_{l != m} 1/(z_m - z_l) prod_{r != m, l} (x - z_r)/(z_m - z_r)
+                double d = 0.0;
+                for (int l = 0; l < 5; ++l) {
+                    if (l == m) continue;
+                    double t = 1.0 / (zl[m] - zl[l]);
+                    for (int r = 0; r < 5; ++r)
+                        if (r != m && r != l) t *= (x - zl[r]) / (zl[m] - zl[r]);
+                    d += t;
+                }
+                w[((size_t)j * nz + k) * 5 + m] = d;
+            }
+        }
+    return w;
+}
+
 }  // namespace islbm

@@ -30,6 +30,11 @@ struct HostGeom {
 // on the device by launch_xi_weights).  Returns an error message or "".
 std::string build_host_geometry(const islbm_params& p, HostGeom& g);
 
+// d/dz weights of the 5-point Lagrange interpolant on the vertical mesh line of (j, k),
+// nodes z[j][n .. n+4] with n = k - 2 clamped to [0, nz - 5], evaluated at z[j][k]:
+// [ny][nz][5].
+std::vector<double> zderiv_weights(const HostGeom& g);
+
 // Device-resident tables in compute precision C.
 template <typename C>
 struct DevTables {
@@ -86,6 +91,16 @@ struct InitScale {
 hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, void* mac, long long n, int nx, int ny, int nz,
                                    double rho0, const double u0[3], double amp, uint64_t seed, const float* scale);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
+
+// Turbulence statistics (islbm_stats.hip): NSTAT FP64 accumulators [a][k][j][i], names in
+// kStatNames; zdw = 5-point wall-normal derivative weights [j][k][5] (zderiv_weights).
+constexpr int NSTAT = 33;
+extern const char* const kStatNames[NSTAT];
+hipError_t launch_stats(hipStream_t st, bool fp64, const void* mac, double* acc, const double* zdw, double dx, double dy,
+                        int nx, int ny, int nz);
+// deterministic sum of n values (compute precision) into out[0]; part holds kSumBlocks partials
+constexpr int kSumBlocks = 512;
+hipError_t launch_sum(hipStream_t st, bool fp64, const void* x, long long n, double* part, double* out);
 
 hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* part,
                              double* acc, int nx, int ny, int nz, int jplane);

@@ -34,7 +34,15 @@ HEADER_SYMBOLS = [
     "islbm_batch_create", "islbm_batch_destroy", "islbm_batch_submit", "islbm_batch_advance",
     "islbm_batch_status", "islbm_batch_result", "islbm_batch_summary", "islbm_batch_bytes_in_use",
     "islbm_batch_last_error", "islbm_batch_now",
+    "islbm_upload_macros", "islbm_get_params", "islbm_mass_correct", "islbm_monitor",
+    "islbm_stats_enable", "islbm_stats_disable", "islbm_stats_accumulate", "islbm_stats_reset", "islbm_stats_count",
+    "islbm_stats_set_count", "islbm_stats_names", "islbm_stats_download", "islbm_stats_upload",
+    "islbm_checkpoint_write", "islbm_checkpoint_read", "islbm_stats_write", "islbm_stats_read",
+    "islbm_monitor_append", "islbm_io_last_error",
 ]
+STAT_NAMES = ["U", "V", "W", "P", "UU", "UV", "UW", "VV", "VW", "WW", "PU", "PV", "PW", "PP", "KT",
+              "DUDX2", "DUDY2", "DUDZ2", "DVDX2", "DVDY2", "DVDZ2", "DWDX2", "DWDY2", "DWDZ2",
+              "UUU", "UUV", "UUW", "VVU", "VVV", "VVW", "WWU", "WWV", "WWW"]
 REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 
@@ -114,6 +122,22 @@ def lib():
     L.islbm_batch_bytes_in_use.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.islbm_batch_last_error.restype = ctypes.c_char_p
     L.islbm_batch_now.restype = ctypes.c_double
+    L.islbm_upload_macros.argtypes = [vp, dp, dp, dp, dp]
+    L.islbm_get_params.argtypes = [vp, pp]
+    L.islbm_mass_correct.argtypes = [vp, ctypes.c_double, dp]
+    L.islbm_monitor.argtypes = [vp, dp]
+    L.islbm_stats_enable.argtypes = [vp, i32]
+    for nm in ("islbm_stats_disable", "islbm_stats_accumulate", "islbm_stats_reset"):
+        getattr(L, nm).argtypes = [vp]
+    L.islbm_stats_count.argtypes = [vp, i64p]
+    L.islbm_stats_set_count.argtypes = [vp, i64]
+    L.islbm_stats_names.argtypes = [ctypes.POINTER(ctypes.c_char_p), i32]
+    L.islbm_stats_download.argtypes = [vp, ctypes.c_char_p, dp]
+    L.islbm_stats_upload.argtypes = [vp, ctypes.c_char_p, dp]
+    for nm in ("islbm_checkpoint_write", "islbm_checkpoint_read", "islbm_stats_write", "islbm_stats_read"):
+        getattr(L, nm).argtypes = [vp, ctypes.c_char_p, i32]
+    L.islbm_monitor_append.argtypes = [vp, ctypes.c_char_p]
+    L.islbm_io_last_error.restype = ctypes.c_char_p
     for name in HEADER_SYMBOLS:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int and name not in ("islbm_last_error",):
@@ -265,6 +289,71 @@ class Solver:
             if n == -1 or -n <= cap:
                 _check(n, f"islbm_get_table({name})")
             cap = -n
+
+    # -- restart, monitor, mass correction -------------------------------------------------
+    def upload_macros(self, rho=None, u=None, v=None, w=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rho, u, v, w)]
+        for a in arrs:
+            if a is not None and a.size != self.n:
+                raise ValueError("macroscopic arrays must hold nx * ny * nz values")
+        ptrs = [None if a is None else _dp(a) for a in arrs]
+        _check(lib().islbm_upload_macros(self.h, *ptrs), "islbm_upload_macros")
+
+    def mass_correct(self, rho0: float = 1.0) -> float:
+        """Sets rho_modify = rho0 - mean(rho); returns the mean density before the correction."""
+        m = ctypes.c_double()
+        _check(lib().islbm_mass_correct(self.h, float(rho0), ctypes.byref(m)), "islbm_mass_correct")
+        return m.value
+
+    def monitor(self):
+        """(v, u, w, force) at the reference's monitor node."""
+        o = np.zeros(4)
+        _check(lib().islbm_monitor(self.h, _dp(o)), "islbm_monitor")
+        return tuple(o.tolist())
+
+    def _io(self, rc: int, what: str):
+        if rc < 0:
+            msg = lib().islbm_io_last_error().decode(errors="replace")
+            inner = lib().islbm_last_error().decode(errors="replace")
+            raise ISLBMError(f"{what} failed ({rc}): {msg} {inner}".strip())
+
+    def checkpoint_write(self, directory: str, rank: int = 0):
+        self._io(lib().islbm_checkpoint_write(self.h, directory.encode(), int(rank)), "islbm_checkpoint_write")
+
+    def checkpoint_read(self, directory: str, rank: int = 0):
+        self._io(lib().islbm_checkpoint_read(self.h, directory.encode(), int(rank)), "islbm_checkpoint_read")
+
+    def monitor_append(self, path: str):
+        self._io(lib().islbm_monitor_append(self.h, path.encode()), "islbm_monitor_append")
+
+    # -- turbulence statistics -------------------------------------------------------------
+    def stats_enable(self, every: int = 1):
+        _check(lib().islbm_stats_enable(self.h, int(every)), "islbm_stats_enable")
+
+    def stats_disable(self):
+        _check(lib().islbm_stats_disable(self.h), "islbm_stats_disable")
+
+    def stats_accumulate(self):
+        _check(lib().islbm_stats_accumulate(self.h), "islbm_stats_accumulate")
+
+    def stats_reset(self):
+        _check(lib().islbm_stats_reset(self.h), "islbm_stats_reset")
+
+    def stats_count(self) -> int:
+        v = ctypes.c_int64()
+        _check(lib().islbm_stats_count(self.h, ctypes.byref(v)), "islbm_stats_count")
+        return v.value
+
+    def stats(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape)
+        _check(lib().islbm_stats_download(self.h, name.encode(), _dp(out)), f"islbm_stats_download({name})")
+        return out
+
+    def stats_write(self, directory: str, rank: int = 0):
+        self._io(lib().islbm_stats_write(self.h, directory.encode(), int(rank)), "islbm_stats_write")
+
+    def stats_read(self, directory: str, rank: int = 0):
+        self._io(lib().islbm_stats_read(self.h, directory.encode(), int(rank)), "islbm_stats_read")
 
     def time_steps(self, n: int):
         """(ms for n steps, summed ms of the stream-collide launches) from HIP events."""

@@ -195,6 +195,7 @@ enum { XI3, XI4, XI5, XI6, XI15, XI16, XI17, XI18, NXI };
 enum { B3, B4, B15, B16, NB };
 enum { BX37, BX38, BY378, BXI378, BX49, BX410, BY4910, BXI4910, BY15, BXI15, BY16, BXI16, NBW };
 
+#define OR_NSTAT 33
 struct or_state {
     or_params p;
     int NX6, NY6, NZ6;
@@ -214,6 +215,9 @@ struct or_state {
     long step;
     int cur;
     double M[NQ][NQ], Mi[NQ][NQ];
+    double* st[OR_NSTAT];                 /* statistics sums, padded [NY6][NZ6][NX6] */
+    double* slp;                          /* d/dz weights [NY6][NZ6][5] */
+    long st_count;
 };
 
 /* D3Q19 d'Humieres moment basis (MRT_Matrix.h:27-46, Krueger p.672), generated from its
@@ -432,6 +436,8 @@ void or_destroy(or_state* s)
     for (int b = 0; b < 2; ++b)
         for (int q = 0; q < NQ; ++q) free(s->f[b][q]);
     free(s->rho); free(s->u); free(s->v); free(s->w); free(s->ub);
+    for (int a = 0; a < OR_NSTAT; ++a) free(s->st[a]);
+    free(s->slp);
     free(s);
 }
 
@@ -915,4 +921,110 @@ int or_get_table(const or_state* s, const char* name, double* out, int cap)
         }
 #undef NEED
     return 0;
+}
+
+/* ---------------------------------------------------------------------------------- */
+/* Turbulence statistics: MeanVars and MeanDerivatives (statistics.h:122-224) over the  */
+/* padded interior I, J, K = 3 .. N6-4, on the ghost-filled macroscopic arrays.          */
+/* The reference's wall-normal weights ZSlopePara (indexed by K alone) are filled by     */
+/* code absent from the repository; here they are the 5-point Lagrange derivative        */
+/* weights on the node's vertical mesh line z[J][n .. n+4], evaluated at z[J][K].         */
+/* ---------------------------------------------------------------------------------- */
+static const char* const OR_STAT_NAMES[OR_NSTAT] = {
+    "U", "V", "W", "P", "UU", "UV", "UW", "VV", "VW", "WW", "PU", "PV", "PW", "PP", "KT",
+    "DUDX2", "DUDY2", "DUDZ2", "DVDX2", "DVDY2", "DVDZ2", "DWDX2", "DWDY2", "DWDZ2",
+    "UUU", "UUV", "UUW", "VVU", "VVV", "VVW", "WWU", "WWV", "WWW"};
+
+void or_stats_enable(or_state* s)
+{
+    const int NY6 = s->NY6, NZ6 = s->NZ6;
+    for (int a = 0; a < OR_NSTAT; ++a) {
+        free(s->st[a]);
+        s->st[a] = zalloc(s->ntot);
+    }
+    free(s->slp);
+    s->slp = zalloc((long)NY6 * NZ6 * 5);
+    for (int J = 0; J < NY6; ++J)
+        for (int K = 3; K < NZ6 - 3; ++K) {
+            int n = K - 2;                       /* statistics.h:196-198 */
+            if (K <= 4) n = 3;
+            if (K >= NZ6 - 5) n = NZ6 - 8;
+            const double* zl = s->z + (long)J * NZ6 + n;
+            const double x = s->z[(long)J * NZ6 + K];
+            for (int m = 0; m < 5; ++m) {
+                double d = 0.0;
+                for (int l = 0; l < 5; ++l) {
+                    if (l == m) continue;
+                    double t = 1.0 / (zl[m] - zl[l]);
+                    for (int r = 0; r < 5; ++r)
+                        if (r != m && r != l) t *= (x - zl[r]) / (zl[m] - zl[r]);
+                    d += t;
+                }
+                s->slp[((long)J * NZ6 + K) * 5 + m] = d;
+            }
+        }
+    s->st_count = 0;
+}
+
+void or_stats_accumulate(or_state* s)
+{
+    const int NX6 = s->NX6, NY6 = s->NY6, NZ6 = s->NZ6;
+    const long nf = s->nface;
+    if (!s->st[0]) or_stats_enable(s);
+#pragma omp parallel for schedule(static)
+    for (int J = 3; J < NY6 - 3; ++J)
+        for (int K = 3; K < NZ6 - 3; ++K)
+            for (int I = 3; I < NX6 - 3; ++I) {
+                const long id = (long)J * nf + (long)K * NX6 + I;
+                const double u1 = s->u[id], v1 = s->v[id], w1 = s->w[id];
+                const double p1 = 1.0 / 3.0 * s->rho[id] - 1.0 / 3.0;
+                double v[OR_NSTAT];
+                v[0] = u1; v[1] = v1; v[2] = w1; v[3] = p1;
+                v[4] = u1 * u1; v[5] = u1 * v1; v[6] = u1 * w1; v[7] = v1 * v1; v[8] = v1 * w1; v[9] = w1 * w1;
+                v[10] = p1 * u1; v[11] = p1 * v1; v[12] = p1 * w1; v[13] = p1 * p1;
+                v[14] = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1);
+                const double* fld[3] = {s->u, s->v, s->w};
+                int n = K - 2;
+                if (K <= 4) n = 3;
+                if (K >= NZ6 - 5) n = NZ6 - 8;
+                const long idz = (long)J * nf + (long)n * NX6 + I;
+                const double* sl = s->slp + ((long)J * NZ6 + K) * 5;
+                for (int c = 0; c < 3; ++c) {
+                    const double* f = fld[c];
+                    const double dx = (8.0 * (f[id + 1] - f[id - 1]) - (f[id + 2] - f[id - 2])) / 12.0 / (s->x[I + 1] - s->x[I]);
+                    const double dy = (8.0 * (f[id + nf] - f[id - nf]) - (f[id + 2 * nf] - f[id - 2 * nf])) / 12.0 /
+                                      (s->y[J + 1] - s->y[J]);
+                    const double dz = f[idz] * sl[0] + f[idz + NX6] * sl[1] + f[idz + 2 * NX6] * sl[2] +
+                                      f[idz + 3 * NX6] * sl[3] + f[idz + 4 * NX6] * sl[4];
+                    v[15 + 3 * c] = dx * dx; v[16 + 3 * c] = dy * dy; v[17 + 3 * c] = dz * dz;
+                }
+                v[24] = u1 * u1 * u1; v[25] = u1 * u1 * v1; v[26] = u1 * u1 * w1;
+                v[27] = v1 * v1 * u1; v[28] = v1 * v1 * v1; v[29] = v1 * v1 * w1;
+                v[30] = w1 * w1 * u1; v[31] = w1 * w1 * v1; v[32] = w1 * w1 * w1;
+                for (int a = 0; a < OR_NSTAT; ++a) s->st[a][id] += v[a];
+            }
+    s->st_count++;
+}
+
+long or_stats_count(const or_state* s) { return s->st_count; }
+
+int or_stats_get(const or_state* s, const char* name, double* out_unique)
+{
+    for (int a = 0; a < OR_NSTAT; ++a)
+        if (strcmp(name, OR_STAT_NAMES[a]) == 0) {
+            if (!s->st[a]) return -1;
+            gather(s, out_unique, s->st[a]);
+            return 0;
+        }
+    return -1;
+}
+
+/* macroscopic fields from unique-node arrays, with the ghost layers refreshed (the state the
+ * statistics read) */
+void or_set_macros(or_state* s, const double* rho, const double* u, const double* v, const double* w)
+{
+    if (rho) scatter(s, s->rho, rho);
+    if (u) scatter(s, s->u, u);
+    if (v) scatter(s, s->v, v);
+    if (w) scatter(s, s->w, w);
 }

@@ -75,6 +75,14 @@ double    or_delta_45(double z_pnt, double y_pnt, double y_zdom, double y_ydom,
                       double ly, double alpha, double minsize);
 double    or_tanh_map(double L, double lattice, double a, int j, int n);
 
+/* Turbulence statistics sums (statistics.h:122-224), unique-node read-out by name
+ * (U V W P UU UV UW VV VW WW PU PV PW PP KT DUDX2 .. DWDZ2 UUU .. WWW). */
+void      or_stats_enable(or_state* s);
+void      or_stats_accumulate(or_state* s);
+long      or_stats_count(const or_state* s);
+int       or_stats_get(const or_state* s, const char* name, double* out_unique);
+void      or_set_macros(or_state* s, const double* rho, const double* u, const double* v, const double* w);
+
 /* D3Q19 MRT matrices as used by the oracle (row-major 19x19). */
 void      or_mrt_matrices(double* M, double* Mi);
 

@@ -79,6 +79,13 @@ def lib():
         L.or_tanh_map.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ip, ip]
         L.or_tanh_map.restype = ctypes.c_double
         L.or_mrt_matrices.argtypes = [dp, dp]
+        L.or_stats_enable.argtypes = [vp]
+        L.or_stats_accumulate.argtypes = [vp]
+        L.or_stats_count.argtypes = [vp]
+        L.or_stats_count.restype = ctypes.c_long
+        L.or_stats_get.argtypes = [vp, ctypes.c_char_p, dp]
+        L.or_stats_get.restype = ip
+        L.or_set_macros.argtypes = [vp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -133,6 +140,27 @@ class Oracle:
         f = np.zeros((19, self.p.nz, self.p.ny, self.p.nx))
         lib().or_get_f(self.h, _dp(f))
         return f
+
+    # statistics (statistics.h:122-224)
+    def stats_enable(self):
+        lib().or_stats_enable(self.h)
+
+    def stats_accumulate(self):
+        lib().or_stats_accumulate(self.h)
+
+    def stats_count(self) -> int:
+        return int(lib().or_stats_count(self.h))
+
+    def stats(self, name: str) -> np.ndarray:
+        p = self.p
+        out = np.empty((p.nz, p.ny, p.nx))
+        if lib().or_stats_get(self.h, name.encode(), _dp(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def set_macros(self, rho, u, v, w):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rho, u, v, w)]
+        lib().or_set_macros(self.h, *[_dp(a) for a in arrs])
 
     def macros(self):
         shape = (self.p.nz, self.p.ny, self.p.nx)

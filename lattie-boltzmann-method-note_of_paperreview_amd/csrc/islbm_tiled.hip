@@ -52,8 +52,9 @@ constexpr int HX = TX + 6;
 // A staged level holds
 //   XY: 7-10 [4][HY][HXW]   X: 11-14 [4][TY][HXW]   Z: 5, 6 [2][TY][TX]   Y: 3 4 15-18 [6][HY][TX]
 // (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).
-template <bool W>
+template <bool W, int VB = 4>  // VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16)
 struct Lay {
+    static constexpr int GV = 16 / VB;  // values per 16-byte granule
     static constexpr int TY = W ? 4 : 2;
     static constexpr int NRW = 2 * TY;  // ring waves
     static constexpr int NCW = TY;      // collision waves
@@ -61,8 +62,8 @@ struct Lay {
     static constexpr int HY = TY + 6;
     static constexpr int Y_SZ = HY * TX;       // y halo only
     static constexpr int HO_SZ = 16 * TX * TY;  // hand-over: directions 3-18 x TX TY columns
-    static constexpr int HXW = W ? 72 : HX;    // x-halo row stride
-    static constexpr int XO = W ? 1 : 0;       // column of x = i0 - 3
+    static constexpr int HXW = W ? TX + 2 * GV : HX;  // x-halo row stride: whole granules from i0 - GV
+    static constexpr int XO = W ? GV - 3 : 0;           // column of x = i0 - 3
     static constexpr int XY_SZ = HY * HXW;
     static constexpr int X_SZ = TY * HXW;
     static constexpr int OFF_XY = 0;
@@ -70,7 +71,7 @@ struct Lay {
     static constexpr int OFF_Z = OFF_X + 4 * X_SZ;
     static constexpr int OFF_Y = W ? OFF_Z + 2 * TY * TX : 4 * XY_SZ;
     static constexpr int USED = W ? OFF_Y + 6 * Y_SZ : OFF_Z + 2 * TY * TX;
-    static constexpr int NW16 = (USED / 4 + 63) / 64;
+    static constexpr int NW16 = (USED / GV + 63) / 64;
     static constexpr int STAGE = W ? NW16 * 256 : USED;
     // staged levels in LDS: a level's loads are issued two steps before it is appended
     static constexpr int NBUF = 3;
@@ -84,12 +85,13 @@ struct Lay {
     static constexpr int LDS_FLOATS = OFF_P012 + NBUF * TY * P_ROW;
     // workgroups resident at once on 256 CUs: the LDS-DMA staging runs at one per CU
     static constexpr int RESIDENT = W ? 256 : 512;
-    static_assert(!W || (OFF_X % 4 == 0 && OFF_Z % 4 == 0 && OFF_Y % 4 == 0 && STAGE % 4 == 0 && OFF_P012 % 4 == 0 &&
-                         P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
+    static_assert(!W || (OFF_X % GV == 0 && OFF_Z % GV == 0 && OFF_Y % GV == 0 && STAGE % 4 == 0 &&
+                         OFF_P012 % 4 == 0 && P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
 };
 // 96 KB (4-byte staging) or 151 KB (16-byte staging): one workgroup per CU
-static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
+static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024 &&
+                  Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
@@ -218,36 +220,48 @@ __device__ inline void lds_dma16(__amdgpu_buffer_rsrc_t r, float* ldst, unsigned
 // Byte offset of staging granule G (16-byte staging) for the tile at (i0, j0): the level's
 // offset goes into the load's scalar offset.  Granules past the staged level (padding of
 // the last load) repeat granule 0.
+template <int VB>
 __device__ inline unsigned granule_offset(int G, int i0, int j0, int nx, int ny, unsigned nbq)
 {
-    using LW = Lay<true>;
-    constexpr int TY = LW::TY, HY = LW::HY, Y_SZ = LW::Y_SZ;
+    using LW = Lay<true, VB>;
+    constexpr int TY = LW::TY, HY = LW::HY, Y_SZ = LW::Y_SZ, GV = LW::GV;
+    constexpr int RH = LW::HXW / GV, RN = TX / GV;  // granules per row with / without x halo
     auto wx = [&](int x) { return ((x % nx) + nx) % nx; };
     auto wy = [&](int y) { return ((y % ny) + ny) % ny; };
     int q, y, x;
-    if (G >= (LW::OFF_Y + 6 * Y_SZ) / 4) G = 0;
-    if (G < LW::OFF_X / 4) {  // 7-10: 18 granules per row, rows j0-3 .. j0+4
-        const int row = G / 18, cg = G % 18;
+    if (G >= (LW::OFF_Y + 6 * Y_SZ) / GV) G = 0;
+    if (G < LW::OFF_X / GV) {  // 7-10: rows j0-3 .. j0+TY+2
+        const int row = G / RH, cg = G % RH;
         q = 7 + row / HY;
         y = wy(j0 - 3 + row % HY);
-        x = wx(i0 - 4 + 4 * cg);
-    } else if (G < LW::OFF_Z / 4) {  // 11-14: rows j0, j0+1
-        const int g = G - LW::OFF_X / 4, row = g / 18, cg = g % 18;
+        x = wx(i0 - GV + GV * cg);
+    } else if (G < LW::OFF_Z / GV) {  // 11-14: rows j0 .. j0+TY-1
+        const int g = G - LW::OFF_X / GV, row = g / RH, cg = g % RH;
         q = 11 + row / TY;
         y = wy(j0 + row % TY);
-        x = wx(i0 - 4 + 4 * cg);
-    } else if (G < LW::OFF_Y / 4) {  // 5, 6 at the nodes
-        const int g = G - LW::OFF_Z / 4;
-        q = 5 + g / (TY * TX / 4);
-        y = wy(j0 + (g / (TX / 4)) % TY);
-        x = wx(i0 + 4 * (g % (TX / 4)));
-    } else {  // 3 4 15-18: rows j0-3 .. j0+4, no x halo
-        const int g = G - LW::OFF_Y / 4, row = g / (TX / 4);
+        x = wx(i0 - GV + GV * cg);
+    } else if (G < LW::OFF_Y / GV) {  // 5, 6 at the nodes
+        const int g = G - LW::OFF_Z / GV;
+        q = 5 + g / (TY * RN);
+        y = wy(j0 + (g / RN) % TY);
+        x = wx(i0 + GV * (g % RN));
+    } else {  // 3 4 15-18: rows j0-3 .. j0+TY+2, no x halo
+        const int g = G - LW::OFF_Y / GV, row = g / RN;
         q = qy(row / HY);
         y = wy(j0 - 3 + row % HY);
-        x = wx(i0 + 4 * (g % (TX / 4)));
+        x = wx(i0 + GV * (g % RN));
     }
-    return (unsigned)q * nbq + 4u * (unsigned)(y * nx + x);
+    return (unsigned)q * nbq + (unsigned)VB * (unsigned)(y * nx + x);
+}
+
+// a 16-bit population code as an FP32 value, before its per-direction scale
+template <Store ST>
+__device__ inline float dec16(unsigned short h)
+{
+    if constexpr (ST == Store::BF16)
+        return __uint_as_float((unsigned)h << 16);
+    else
+        return (float)__builtin_bit_cast(_Float16, h);
 }
 
 template <Store ST, bool W>
@@ -257,10 +271,9 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     using CD = Codec<ST>;
     using raw = typename CD::raw;
     using IO = BufIO<ST>;
-    using LY = Lay<W>;
-    constexpr bool k16 = (ST != Store::F32);
-    static_assert(!W || !k16, "16-byte staging is for FP32 storage");
     constexpr unsigned SZ = sizeof(raw);
+    using LY = Lay<W, W ? (int)SZ : 4>;
+    constexpr bool k16 = (ST != Store::F32);
     constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
     constexpr int OFF_YP = LY::OFF_YP, OFF_HO = LY::OFF_HO, OFF_XIB = LY::OFF_XIB, OFF_P012 = LY::OFF_P012;
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         if constexpr (W) {
 #pragma unroll
             for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m)
-                g16[m] = granule_offset(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
+                g16[m] = granule_offset<W ? (int)SZ : 4>(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
         }
         float wx[2][7], wy[2][7];
 #pragma unroll
@@ -378,7 +391,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         };
         // 16-bit storage: decode this wave's share to FP32 deviations in place
         auto decode_region = [&](float* R, int n, float isc) {
-            if constexpr (k16) {
+            if constexpr (k16 && !W) {
                 for (int e = lane; e < n; e += 64) {
                     const unsigned short code = (unsigned short)(__float_as_uint(R[e]) & 0xFFFFu);
                     const raw r = __builtin_bit_cast(raw, code);
@@ -387,7 +400,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             }
         };
         auto decode = [&](int b) {
-            if constexpr (k16) {
+            if constexpr (k16 && !W) {
                 float* L = lds + b * STAGE;
                 decode_region(L + OFF_XY + w * XY_SZ, XY_SZ, a.inv_scale[7 + w]);
                 decode_region(L + OFF_X + w * X_SZ, X_SZ, a.inv_scale[11 + w]);
@@ -422,6 +435,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                 // the counts a ring wave can leave outstanding (0 waits for everything)
                 switch (outstanding) {
                 case 2: __builtin_amdgcn_s_waitcnt(vmcnt_imm(2)); break;
+                case 3: __builtin_amdgcn_s_waitcnt(vmcnt_imm(3)); break;
                 case 4: __builtin_amdgcn_s_waitcnt(vmcnt_imm(4)); break;
                 case 5: __builtin_amdgcn_s_waitcnt(vmcnt_imm(5)); break;
                 case 6: __builtin_amdgcn_s_waitcnt(vmcnt_imm(6)); break;
@@ -443,10 +457,12 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             const auto rxi = rsrc_of(a.t.xiw, xi_bytes);
             unsigned p16 = 0;  // 16-byte staging of directions 0-2 (group B): this lane's granule
             if constexpr (G == 1 && W) {
-                const int g = lane < 52 ? lane : 0;
-                const int q = g < 16 ? 0 : (g < 34 ? 1 : 2);
-                const int x = g < 16 ? i0 + 4 * g : i0 - 4 + 4 * (g - (q == 1 ? 16 : 34));
-                p16 = (unsigned)q * nbq + 4u * (unsigned)(jw * nx + ((x % nx) + nx) % nx);
+                // granules: direction 0 at the nodes, then 1 and 2 with the x halo from i0 - GV
+                constexpr int GV = LY::GV, n0 = TX / GV, n12 = HXW / GV;
+                const int g = lane < n0 + 2 * n12 ? lane : 0;
+                const int q = g < n0 ? 0 : (g < n0 + n12 ? 1 : 2);
+                const int x = g < n0 ? i0 + GV * g : i0 - GV + GV * (g - (q == 1 ? n0 : n0 + n12));
+                p16 = (unsigned)q * nbq + SZ * (unsigned)(jw * nx + ((x % nx) + nx) % nx);
             }
             auto fetch_xi = [&](int tk, int par) {
                 if constexpr (G == 1)  // an FP32 table whatever the population storage
@@ -519,17 +535,40 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                 }
                 outstanding = ntab + (adv ? nbatch : 0);
                 if (adv) {
-                    // horizontal filters of this group's directions; shift the ring and append
-                    auto xf = [&](const float* src, const float* wv) {
+                    // horizontal filters of this group's directions; shift the ring and append.
+                    // Staged value at offset o of level x: FP32 as stored, 16-bit codes
+                    // (16-byte staging) converted here; the direction's scale multiplies the
+                    // filtered value (the filters are linear).
+                    constexpr bool raw16 = W && k16;
+                    const unsigned short* S16 = (const unsigned short*)S;
+                    auto sv = [&](int o) -> float {
+                        if constexpr (raw16)
+                            return dec16<ST>(S16[o]);
+                        else
+                            return S[o];
+                    };
+                    auto scl = [&](float v, int q) -> float {
+                        if constexpr (raw16)
+                            return v * a.inv_scale[q];
+                        else
+                            return v;
+                    };
+                    auto xf = [&](int o, const float* wv) {
+                        float sum = sv(o) * wv[0];
+#pragma unroll
+                        for (int l = 1; l < 7; ++l) sum = sum + sv(o + l) * wv[l];
+                        return sum;
+                    };
+                    auto xfp = [&](const float* src, const float* wv) {
                         float sum = src[0] * wv[0];
 #pragma unroll
                         for (int l = 1; l < 7; ++l) sum = sum + src[l] * wv[l];
                         return sum;
                     };
-                    auto yf = [&](const float* src, const float* wv) {
-                        float sum = src[0] * wv[0];
+                    auto yf = [&](int o, const float* wv) {
+                        float sum = sv(o) * wv[0];
 #pragma unroll
-                        for (int n = 1; n < 7; ++n) sum = sum + src[n * TX] * wv[n];
+                        for (int n = 1; n < 7; ++n) sum = sum + sv(o + n * TX) * wv[n];
                         return sum;
                     };
 #pragma unroll
@@ -544,31 +583,31 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                         for (int d = 0; d < 4; ++d) {
                             const float* wyy = wy[ysel(7 + d)];
                             for (int c = lane; c < HXW; c += 64) {
-                                const float* src = S + OFF_XY + d * XY_SZ + ty * HXW + c;
-                                float sum = src[0] * wyy[0];
+                                const int o = OFF_XY + d * XY_SZ + ty * HXW + c;
+                                float sum = sv(o) * wyy[0];
 #pragma unroll
-                                for (int n = 1; n < 7; ++n) sum = sum + src[n * HXW] * wyy[n];
+                                for (int n = 1; n < 7; ++n) sum = sum + sv(o + n * HXW) * wyy[n];
                                 YP[d * HXW + c] = sum;
                             }
                         }
-                        g[0][6] = S[OFF_Z + ty * TX + tx];         // 5 at the node (no filter)
-                        g[1][6] = S[OFF_Z + (TY + ty) * TX + tx];  // 6
+                        g[0][6] = scl(sv(OFF_Z + ty * TX + tx), 5);         // 5 at the node (no filter)
+                        g[1][6] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);  // 6
 #pragma unroll
                         for (int d = 0; d < 4; ++d)  // 7-10: x pass over the y-passed row
-                            g[2 + d][6] = xf(YP + d * HXW + XO + tx, wx[xsel(7 + d)]);
+                            g[2 + d][6] = scl(xfp(YP + d * HXW + XO + tx, wx[xsel(7 + d)]), 7 + d);
 #pragma unroll
                         for (int d = 0; d < 2; ++d)  // 11, 12: x filter
-                            g[6 + d][6] = xf(S + OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]);
+                            g[6 + d][6] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
                     } else {
 #pragma unroll
                         for (int d = 0; d < 2; ++d)  // 3, 4: y filter
-                            g[d][6] = yf(S + OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(3 + d)]);
+                            g[d][6] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(3 + d)]), 3 + d);
 #pragma unroll
                         for (int d = 2; d < 4; ++d)  // 13, 14: x filter
-                            g[d][6] = xf(S + OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]);
+                            g[d][6] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
 #pragma unroll
                         for (int d = 2; d < 6; ++d)  // 15-18: y filter
-                            g[2 + d][6] = yf(S + OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]);
+                            g[2 + d][6] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]), qy(d));
                     }
                 }
                 if (ho >= 0) {
@@ -637,6 +676,22 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             }
         };
         auto reduce012 = [&](float (&F)[NQ], const float* P) {
+            if constexpr (W && k16) {
+                // 16-bit codes: direction 0 at the nodes, 1 and 2 with the x halo
+                const unsigned short* P16 = (const unsigned short*)P;
+                const unsigned short* Q1 = P16 + TX + XO + tx;
+                const unsigned short* Q2 = P16 + TX + HXW + XO + tx;
+                F[0] = dec16<ST>(P16[tx]) * isc0;
+                float s1 = dec16<ST>(Q1[0]) * wx0[0], s2 = dec16<ST>(Q2[0]) * wx2[0];
+#pragma unroll
+                for (int l = 1; l < 7; ++l) {
+                    s1 = s1 + dec16<ST>(Q1[l]) * wx0[l];
+                    s2 = s2 + dec16<ST>(Q2[l]) * wx2[l];
+                }
+                F[1] = s1 * isc1;
+                F[2] = s2 * isc2;
+                return;
+            }
             F[0] = dec(P[tx], isc0);
             const float* P1 = P + TX + XO + tx;
             const float* P2 = P + TX + HXW + XO + tx;
@@ -729,8 +784,9 @@ static hipError_t launch_t(hipStream_t st, const StepLaunch& L)
 
 bool tiled_wide_staging(const LaunchGeom& g, Store store)
 {
-    // whole 16-byte granules of a row never straddle the periodic wrap in x
-    return store == Store::F32 && g.nx % 4 == 0;
+    // whole 16-byte granules of a row (4 FP32 or 8 16-bit values) never straddle the
+    // periodic wrap in x
+    return store == Store::F32 ? g.nx % 4 == 0 : g.nx % 8 == 0;
 }
 
 hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
@@ -743,8 +799,10 @@ hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
     switch (L.store) {
     case Store::F32:
         return tiled_wide_staging(L.g, L.store) ? launch_t<Store::F32, true>(st, L) : launch_t<Store::F32, false>(st, L);
-    case Store::BF16: return launch_t<Store::BF16, false>(st, L);
-    case Store::FP16: return launch_t<Store::FP16, false>(st, L);
+    case Store::BF16:
+        return tiled_wide_staging(L.g, L.store) ? launch_t<Store::BF16, true>(st, L) : launch_t<Store::BF16, false>(st, L);
+    case Store::FP16:
+        return tiled_wide_staging(L.g, L.store) ? launch_t<Store::FP16, true>(st, L) : launch_t<Store::FP16, false>(st, L);
     default: return hipErrorNotSupported;
     }
 }

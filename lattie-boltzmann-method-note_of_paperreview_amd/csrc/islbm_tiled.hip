@@ -52,7 +52,9 @@ constexpr int HX = TX + 6;
 // A staged level holds
 //   XY: 7-10 [4][HY][HXW]   X: 11-14 [4][TY][HXW]   Z: 5, 6 [2][TY][TX]   Y: 3 4 15-18 [6][HY][TX]
 // (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).
-template <bool W, int VB = 4>  // VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16)
+// VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16); TS: transposed result
+// stores (BF16 with 16-byte staging)
+template <bool W, int VB = 4, bool TS = false>
 struct Lay {
     static constexpr int GV = 16 / VB;  // values per 16-byte granule
     static constexpr int TY = W ? 4 : 2;
@@ -82,7 +84,11 @@ struct Lay {
     // directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
     static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
     static constexpr int OFF_P012 = OFF_XIB + NBUF * TY * 64;
-    static constexpr int LDS_FLOATS = OFF_P012 + NBUF * TY * P_ROW;
+    // transposed result stores: an area per collision wave, the 19 codes of its 64 nodes as
+    // 19 rows of 64, then rho u v w as 4 rows of 64 floats
+    static constexpr int OFF_ST = OFF_P012 + NBUF * TY * P_ROW;
+    static constexpr int ST_SZ = (W && VB == 2 && TS) ? 19 * TX / 2 + 4 * TX : 0;
+    static constexpr int LDS_FLOATS = OFF_ST + TY * ST_SZ;
     // workgroups resident at once on 256 CUs: the LDS-DMA staging runs at one per CU
     static constexpr int RESIDENT = W ? 256 : 512;
     static_assert(!W || (OFF_X % GV == 0 && OFF_Z % GV == 0 && OFF_Y % GV == 0 && STAGE % 4 == 0 &&
@@ -91,7 +97,7 @@ struct Lay {
 };
 // 96 KB (4-byte staging) or 151 KB (16-byte staging): one workgroup per CU
 static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024 &&
-                  Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
+                  Lay<true, 2, true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
@@ -272,7 +278,10 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     using raw = typename CD::raw;
     using IO = BufIO<ST>;
     constexpr unsigned SZ = sizeof(raw);
-    using LY = Lay<W, W ? (int)SZ : 4>;
+    // BF16 results leave through 16-byte stores transposed in LDS (measured at 128 x 256 x 128:
+    // faster for BF16, slower for FP16 and FP32, which store value by value)
+    constexpr bool TS = W && ST == Store::BF16;
+    using LY = Lay<W, W ? (int)SZ : 4, TS>;
     constexpr bool k16 = (ST != Store::F32);
     constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
@@ -416,11 +425,16 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
 
         auto ring = [&](auto gc) {
             constexpr int G = decltype(gc)::value;
-            float g[8][7];
+            // The ring of direction d holds its last 7 appended levels in 8 slots, four
+            // register pairs h[d][0..3].  Appends alternate in parity: an even append writes
+            // slot 7 (window: slots 1-7); an odd append moves the pairs down by one (three
+            // 64-bit moves) and writes slot 6 (window: slots 0-6).  The step body is compiled
+            // for both parities, so no append shifts the ring value by value.
+            float h[8][8];
 #pragma unroll
             for (int d = 0; d < 8; ++d)
 #pragma unroll
-                for (int m = 0; m < 7; ++m) g[d][m] = 0.f;
+                for (int m = 0; m < 8; ++m) h[d][m] = 0.f;
             // Every vector-memory instruction of a ring wave is a staging load into LDS, issued
             // two steps before it is read.  In step s, group B stages the xi weights of the
             // target handed over in step s+2 and the rows of 0-2 of the target handed over in
@@ -512,7 +526,10 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             }
             fetch(min(L0 + 1, L1), 1);
             int outstanding = ntab + nbatch;  // loads issued in the step before
-            while (sched_step(sc, k1, nz, adv, ho)) {
+            // step s has parity P = s % 2: the loop below alternates the two forms
+            auto step = [&](auto pc) -> bool {
+                constexpr int P = decltype(pc)::value;
+                if (!sched_step(sc, k1, nz, adv, ho)) return false;
                 const int x = sc.top;  // level appended in this step (if adv)
                 // this wave's share of level x and of the rows of directions 0-2 and xi weights
                 // staged for this step are in LDS
@@ -534,8 +551,9 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                     b = b + 1 == NBUF ? 0 : b + 1;
                 }
                 outstanding = ntab + (adv ? nbatch : 0);
+                float nv[8];  // this step's filtered values of the group's directions
                 if (adv) {
-                    // horizontal filters of this group's directions; shift the ring and append.
+                    // horizontal filters of this group's directions.
                     // Staged value at offset o of level x: FP32 as stored, 16-bit codes
                     // (16-byte staging) converted here; the direction's scale multiplies the
                     // filtered value (the filters are linear).
@@ -571,10 +589,6 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                         for (int n = 1; n < 7; ++n) sum = sum + sv(o + n * TX) * wv[n];
                         return sum;
                     };
-#pragma unroll
-                    for (int d = 0; d < 8; ++d)
-#pragma unroll
-                        for (int m = 0; m < 6; ++m) g[d][m] = g[d][m + 1];
                     if constexpr (G == 0) {
                         // y pass of 7-10 on this wave's row, all HX columns, into its own LDS
                         // area (read back by this wave only: no barrier needed)
@@ -590,41 +604,75 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                                 YP[d * HXW + c] = sum;
                             }
                         }
-                        g[0][6] = scl(sv(OFF_Z + ty * TX + tx), 5);         // 5 at the node (no filter)
-                        g[1][6] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);  // 6
+                        nv[0] = scl(sv(OFF_Z + ty * TX + tx), 5);         // 5 at the node (no filter)
+                        nv[1] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);  // 6
 #pragma unroll
                         for (int d = 0; d < 4; ++d)  // 7-10: x pass over the y-passed row
-                            g[2 + d][6] = scl(xfp(YP + d * HXW + XO + tx, wx[xsel(7 + d)]), 7 + d);
+                            nv[2 + d] = scl(xfp(YP + d * HXW + XO + tx, wx[xsel(7 + d)]), 7 + d);
 #pragma unroll
                         for (int d = 0; d < 2; ++d)  // 11, 12: x filter
-                            g[6 + d][6] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
+                            nv[6 + d] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
                     } else {
 #pragma unroll
                         for (int d = 0; d < 2; ++d)  // 3, 4: y filter
-                            g[d][6] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(3 + d)]), 3 + d);
+                            nv[d] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(3 + d)]), 3 + d);
 #pragma unroll
                         for (int d = 2; d < 4; ++d)  // 13, 14: x filter
-                            g[d][6] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
+                            nv[d] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
 #pragma unroll
                         for (int d = 2; d < 6; ++d)  // 15-18: y filter
-                            g[2 + d][6] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]), qy(d));
+                            nv[2 + d] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]), qy(d));
                     }
                 }
-                if (ho >= 0) {
+                // hand-over of target ho from the window of the last append (parity S)
+                auto handover = [&](auto wc) {
+                    constexpr int S = decltype(wc)::value;
                     // xi weights of the target (wave-uniform LDS reads, staged in step s-1)
                     const float* XI = lds + OFF_XIB + ((s % NBUF) * TY + ty) * 64;
                     float* HO = lds + OFF_HO + (s & 1) * HO_SZ;
+                    auto win = [&](int d, int m) -> float { return h[d][m + (S == 0 ? 1 : 0)]; };  // S = P
 #pragma unroll
                     for (int d = 0; d < 8; ++d) {
                         const int q = gq(G, d);
                         const float* wz = XI + xifam(q) * 7;
-                        float sum = g[d][0] * wz[0];
+                        float sum = win(d, 0) * wz[0];
 #pragma unroll
-                        for (int m = 1; m < 7; ++m) sum = sum + g[d][m] * wz[m];
+                        for (int m = 1; m < 7; ++m) sum = sum + win(d, m) * wz[m];
                         HO[(q - 3) * (TX * TY) + col] = sum;
                     }
+                };
+                // after step s the window is that of parity s % 2: an appending step writes
+                // its slot (odd: after moving the pairs down), a step that appends nothing
+                // moves the window by one slot instead (the first and last levels of the
+                // column only)
+#pragma unroll
+                for (int d = 0; d < 8; ++d) {
+                    if (adv) {
+                        if constexpr (P == 1) {
+#pragma unroll
+                            for (int m = 0; m < 6; ++m) h[d][m] = h[d][m + 2];
+                            h[d][6] = nv[d];
+                        } else {
+                            h[d][7] = nv[d];
+                        }
+                    } else {
+                        if constexpr (P == 1) {
+#pragma unroll
+                            for (int m = 0; m < 7; ++m) h[d][m] = h[d][m + 1];
+                        } else {
+#pragma unroll
+                            for (int m = 7; m > 0; --m) h[d][m] = h[d][m - 1];
+                        }
+                    }
                 }
+                if (ho >= 0) handover(std::integral_constant<int, P>{});
                 ++s;
+                return true;
+            };
+            // the ring starts empty (zeros), as after an odd step
+            for (;;) {
+                if (!step(std::integral_constant<int, 0>{})) break;
+                if (!step(std::integral_constant<int, 1>{})) break;
             }
             // the final step, in which the collision waves take the last target: its rows of
             // 0-2 must be in LDS before the barrier, and no staging load may still write LDS
@@ -665,6 +713,12 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         const auto rfn = rsrc_of(a.fn, NQ * nbq);
         // rho, u, v, w are consecutive arrays of one allocation (checked at launch)
         const auto rmac = rsrc_of(a.rho, 4 * fbytes);
+        // transposed result stores: lane l writes granule l % 8 of population row 8 n + l / 8
+        // of store n (columns i0 + 8 (l % 8) .. + 7, wrapped as a whole since nx % 8 == 0) and
+        // granule l % 16 of macro row l / 16 (columns i0 + 4 (l % 16) .. + 3); the duplicate
+        // columns of a partial tile store their owner's values
+        const unsigned st16 = (unsigned)(lane / 8) * nbq + SZ * (unsigned)((i0 + 8 * (lane % 8)) % nx);
+        const unsigned st16m = (unsigned)(lane / 16) * fbytes + 4u * (unsigned)((i0 + 4 * (lane % 16)) % nx);
         // directions 0, 1, 2 of the target (no z dependence), staged in LDS by the ring
         // waves: the node's own 0 and the 7 x taps of 1 and 2, reduced to F0, F1, F2
         auto dec = [&](float v, float isc) -> float {
@@ -725,9 +779,39 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             collide_pairs<true>(F, rho_mod, (float)a.dt, force, delta, a.les, (float)a.tau, (float)a.cs, rho, ux, uy, uz,
                                 tau_eff);
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
+            const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
+            if constexpr (TS) {
+                // this lane's column of every result into the wave's transposed area, then
+                // 16-byte granules back out: the 19 directions (8 codes of one direction per
+                // granule) in 3 stores, the last with 24 lanes, and rho u v w in one
+                float* T = lds + LY::OFF_ST + ty * LY::ST_SZ;
+                unsigned short* T16 = (unsigned short*)T;
+                float* M = T + NQ * TX / 2;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    raw c;
+                    CD::st(&c, 0, F[q], a.scale[q]);
+                    T16[q * TX + tx] = __builtin_bit_cast(unsigned short, c);
+                }
+                M[tx] = rho;
+                M[TX + tx] = ux;
+                M[2 * TX + tx] = uy;
+                M[3 * TX + tx] = uz;
+                __builtin_amdgcn_wave_barrier();
+                using v4u = unsigned __attribute__((ext_vector_type(4)));
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    const v4u v = *(const v4u*)(T + 4 * (64 * n + lane));
+                    if (n < 2 || lane < 24)
+                        __builtin_amdgcn_raw_buffer_store_b128(v, rfn, (int)st16, (int)(8u * n * nbq + lv + SZ * row),
+                                                               kStoreAux);
+                }
+                const v4u m = *(const v4u*)(M + 4 * lane);
+                __builtin_amdgcn_raw_buffer_store_b128(m, rmac, (int)st16m, (int)(lf + 4u * row), kStoreAux);
+                return;
+            }
 #pragma unroll
             for (int q = 0; q < NQ; ++q) IO::st(rfn, own_b, qoff(q) + lv, F[q], a.scale[q]);
-            const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rmac, (int)own4, (int)lf, kStoreAux);
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), rmac, (int)own4, (int)(lf + fbytes), kStoreAux);
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rmac, (int)own4, (int)(lf + 2 * fbytes), kStoreAux);

@@ -87,10 +87,25 @@ size_t table_bytes(const islbm_params& p)
     return b;
 }
 
+}  // namespace
+
+long long islbm::pop_stride(long long n, int bytes)
+{
+    constexpr long long MiB = 1LL << 20;
+    const long long b = n * bytes;
+    if (b < 4 * MiB) return n;
+    long long m = (b + MiB - 1) / MiB;
+    if (m % 2 == 0) ++m;
+    return m * MiB / bytes;
+}
+
+namespace {
+
 size_t state_bytes(const islbm_params& p)
 {
     const size_t n = (size_t)p.nx * p.ny * p.nz;
-    return 2 * NQ * n * store_bytes(p.precision) + 4 * n * compute_bytes(p.precision) + table_bytes(p);
+    const size_t qs = (size_t)pop_stride((long long)n, (int)store_bytes(p.precision));
+    return 2 * NQ * qs * store_bytes(p.precision) + 4 * n * compute_bytes(p.precision) + table_bytes(p);
 }
 
 __global__ void k_cast_f64(const double* in, float* out, long long n)
@@ -109,6 +124,7 @@ struct islbm_solver {
     int kernel = ISLBM_KERNEL_PERNODE;
     hipStream_t stream = nullptr;
     long long n = 0;
+    long long qs = 0;  // population stride (pop_stride)
     void* f[2] = {nullptr, nullptr};
     int cur = 0;
     void* mac[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -215,20 +231,27 @@ int upload_tables(islbm_solver* s)
     return ISLBM_OK;
 }
 
-// host FP64 view <-> device storage of one population buffer
+// host FP64 view [19][n] <-> device storage of one population buffer [19][qs]
+hipError_t copy_pops(islbm_solver* s, void* dst, const void* src, size_t vb, hipMemcpyKind kind)
+{
+    const size_t row = (size_t)s->n * vb, pitch = (size_t)s->qs * vb;
+    if (kind == hipMemcpyHostToDevice) return hipMemcpy2D(dst, pitch, src, row, row, NQ, kind);
+    return hipMemcpy2D(dst, row, src, pitch, row, NQ, kind);
+}
+
 int put_f(islbm_solver* s, int buf, const double* f)
 {
     const long long n = s->n;
     const size_t tot = (size_t)NQ * n;
     if (s->store == Store::F64) {
-        HIPCHK(hipMemcpy(s->f[buf], f, tot * 8, hipMemcpyHostToDevice));
+        HIPCHK(copy_pops(s, s->f[buf], f, 8, hipMemcpyHostToDevice));
         return ISLBM_OK;
     }
     if (s->store == Store::F32) {
         std::vector<float> h(tot);
         for (int q = 0; q < NQ; ++q)
             for (long long t = 0; t < n; ++t) h[q * n + t] = (float)(f[q * n + t] - wq(q));
-        HIPCHK(hipMemcpy(s->f[buf], h.data(), tot * 4, hipMemcpyHostToDevice));
+        HIPCHK(copy_pops(s, s->f[buf], h.data(), 4, hipMemcpyHostToDevice));
         return ISLBM_OK;
     }
     std::vector<uint16_t> h(tot);
@@ -244,7 +267,7 @@ int put_f(islbm_solver* s, int buf, const double* f)
             }
         }
     }
-    HIPCHK(hipMemcpy(s->f[buf], h.data(), tot * 2, hipMemcpyHostToDevice));
+    HIPCHK(copy_pops(s, s->f[buf], h.data(), 2, hipMemcpyHostToDevice));
     return ISLBM_OK;
 }
 
@@ -254,18 +277,18 @@ int get_f(islbm_solver* s, int buf, double* f)
     const size_t tot = (size_t)NQ * n;
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->store == Store::F64) {
-        HIPCHK(hipMemcpy(f, s->f[buf], tot * 8, hipMemcpyDeviceToHost));
+        HIPCHK(copy_pops(s, f, s->f[buf], 8, hipMemcpyDeviceToHost));
         return ISLBM_OK;
     }
     if (s->store == Store::F32) {
         std::vector<float> h(tot);
-        HIPCHK(hipMemcpy(h.data(), s->f[buf], tot * 4, hipMemcpyDeviceToHost));
+        HIPCHK(copy_pops(s, h.data(), s->f[buf], 4, hipMemcpyDeviceToHost));
         for (int q = 0; q < NQ; ++q)
             for (long long t = 0; t < n; ++t) f[q * n + t] = (double)h[q * n + t] + wq(q);
         return ISLBM_OK;
     }
     std::vector<uint16_t> h(tot);
-    HIPCHK(hipMemcpy(h.data(), s->f[buf], tot * 2, hipMemcpyDeviceToHost));
+    HIPCHK(copy_pops(s, h.data(), s->f[buf], 2, hipMemcpyDeviceToHost));
     for (int q = 0; q < NQ; ++q) {
         const double isc = 1.0 / (double)s->scale[q];
         for (long long t = 0; t < n; ++t) {
@@ -289,7 +312,7 @@ StepLaunch make_launch(islbm_solver* s)
     L.store = s->store;
     L.fp64 = s->fp64;
     L.kernel = s->kernel;
-    L.g = LaunchGeom{s->p.nx, s->p.ny, s->p.nz, s->n};
+    L.g = LaunchGeom{s->p.nx, s->p.ny, s->p.nz, s->n, s->qs};
     L.fo = s->f[s->cur];
     L.fn = s->f[1 - s->cur];
     L.rho = s->mac[0];
@@ -388,10 +411,11 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
         return fail(ISLBM_ERR_BUDGET, "solve needs %zu bytes, budget %zu", s->bytes, budget);
     }
     s->n = (long long)s->p.nx * s->p.ny * s->p.nz;
+    s->qs = pop_stride(s->n, (int)store_bytes(s->p.precision));
     s->store = store_of(s->p.precision);
     s->fp64 = s->p.precision == ISLBM_FP64;
     for (int q = 0; q < NQ; ++q) s->scale[q] = dev_scale(s->p.precision, q);
-    const LaunchGeom lg{s->p.nx, s->p.ny, s->p.nz, s->n};
+    const LaunchGeom lg{s->p.nx, s->p.ny, s->p.nz, s->n, s->qs};
     if (s->p.kernel == ISLBM_KERNEL_AUTO)
         s->kernel = tiled_supported(lg, s->store, s->fp64) ? ISLBM_KERNEL_TILED : ISLBM_KERNEL_PERNODE;
     else
@@ -408,7 +432,7 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
     if (hipSetDevice(s->p.device) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipSetDevice(%d) failed", s->p.device));
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(ISLBM_ERR_HIP, "hipStreamCreate failed"));
-    const size_t fb = (size_t)NQ * s->n * store_bytes(s->p.precision);
+    const size_t fb = (size_t)NQ * s->qs * store_bytes(s->p.precision);
     const size_t mb = (size_t)s->n * compute_bytes(s->p.precision);
     for (int b = 0; b < 2; ++b) {
         if (hipMalloc(&s->f[b], fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc f (%zu bytes)", fb));
@@ -445,8 +469,8 @@ int islbm_init_equilibrium(islbm_solver* s, double rho0, const double u0[3], dou
     HIPCHK(hipSetDevice(s->p.device));
     // computed on the device (k_init_equilibrium): no host arrays, no upload
     s->cur = 0;
-    HIPCHK(launch_init_equilibrium(s->stream, s->store, s->f[0], s->mac[0], s->n, s->p.nx, s->p.ny, s->p.nz, rho0, u0,
-                                   amp, seed, s->scale));
+    HIPCHK(launch_init_equilibrium(s->stream, s->store, s->f[0], s->qs, s->mac[0], s->n, s->p.nx, s->p.ny, s->p.nz,
+                                   rho0, u0, amp, seed, s->scale));
     const double sc[3] = {s->g.force0, 0.0, 0.0};
     HIPCHK(hipMemcpyAsync(s->scal, sc, sizeof sc, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1018,7 +1042,7 @@ void Launch_CollisionStreaming(double* f_old[19], double* f_new[19])
     PtrSet pad{}, uni{};
     for (int q = 0; q < NQ; ++q) {
         pad.p[q] = f_old[q];
-        uni.p[q] = (double*)s->f[s->cur] + (long long)q * n;
+        uni.p[q] = (double*)s->f[s->cur] + (long long)q * s->qs;
     }
     // macroscopic state (u, v, w, rho_d): the flux probe reads v of the previous step
     const int mac_of[4] = {1, 2, 3, 0};  // unique order rho,u,v,w <- padded order u,v,w,rho
@@ -1040,7 +1064,7 @@ void Launch_CollisionStreaming(double* f_old[19], double* f_new[19])
     PtrSet pn{}, un{};
     for (int q = 0; q < NQ; ++q) {
         pn.p[q] = f_new[q];
-        un.p[q] = (double*)s->f[s->cur] + (long long)q * n;
+        un.p[q] = (double*)s->f[s->cur] + (long long)q * s->qs;
     }
     for (int a = 0; a < 4; ++a) {
         pn.p[NQ + a] = g_ref_mac[a];

@@ -55,8 +55,13 @@ enum class Store { F64, F32, BF16, FP16 };
 
 struct LaunchGeom {
     int nx, ny, nz;
-    long long n;  // nodes
+    long long n;   // nodes
+    long long qs;  // population stride: nodes from the array of direction q to that of q + 1 (>= n)
 };
+// Population stride of a grid of n nodes stored in `bytes`-byte values: n, or for arrays of
+// 4 MiB and more, n rounded up to an odd number of MiB (strides that are even multiples of
+// large powers of two, or not multiples of 1 MiB, run the stream-collide step measurably slower).
+long long pop_stride(long long n, int bytes);
 
 // Kernel launchers (islbm_kernels.hip).
 hipError_t launch_xi_weights(hipStream_t st, double* out, const double* zc, const double* y,
@@ -88,8 +93,9 @@ struct InitScale {
 };
 // f = f_eq(rho0, u0 + seeded delta) in the population format and rho, u, v, w (the four
 // consecutive arrays at mac) in the compute precision; scale = 16-bit scale per direction.
-hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, void* mac, long long n, int nx, int ny, int nz,
-                                   double rho0, const double u0[3], double amp, uint64_t seed, const float* scale);
+hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, long long qs, void* mac, long long n, int nx,
+                                   int ny, int nz, double rho0, const double u0[3], double amp, uint64_t seed,
+                                   const float* scale);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
 
 // Turbulence statistics (islbm_stats.hip): NSTAT FP64 accumulators [a][k][j][i], names in

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_collide_pernode(StepArgs<ST> a)
     const C* Y2 = a.t.yw2 + j * 7;
     const C* XI = a.t.xiw + ((long long)k * ny + j) * (NXI * 7);
 
-    auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * N, a.inv_scale[q]}; };
+    auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
 
     C F[NQ];
     F[0] = pop(0)(node);
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_collide_pernode(StepArgs<ST> a)
                            a.les, (C)a.tau, (C)a.cs, rho, ux, uy, uz, tau_eff);
 
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) CD::st(a.fn + (long long)q * N, node, F[q], a.scale[q]);
+    for (int q = 0; q < NQ; ++q) CD::st(a.fn + (long long)q * a.qs, node, F[q], a.scale[q]);
     a.rho[node] = rho;
     a.u[node] = ux;
     a.v[node] = uy;
@@ -288,9 +288,9 @@ __device__ inline uint64_t splitmix64(uint64_t z)
 }
 
 template <Store ST>
-__global__ __launch_bounds__(256) void k_init_equilibrium(void* f, void* mac, long long n, int nx, int ny, int nz,
-                                                          double rho0, double u0x, double u0y, double u0z, double amp,
-                                                          uint64_t seed, InitScale sc)
+__global__ __launch_bounds__(256) void k_init_equilibrium(void* f, long long qs, void* mac, long long n, int nx, int ny,
+                                                          int nz, double rho0, double u0x, double u0y, double u0z,
+                                                          double amp, uint64_t seed, InitScale sc)
 {
     using CD = Codec<ST>;
     using raw = typename CD::raw;
@@ -317,13 +317,13 @@ __global__ __launch_bounds__(256) void k_init_equilibrium(void* f, void* mac, lo
         const double fq = q == 0 ? wq(0) * rho0 * (1.0 - 1.5 * udot)
                                  : wq(q) * rho0 * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * udot);
         if constexpr (ST == Store::F64) {
-            F[(long long)q * n + id] = fq;
+            F[(long long)q * qs + id] = fq;
         } else if constexpr (ST == Store::F32) {
-            F[(long long)q * n + id] = (float)(fq - wq(q));
+            F[(long long)q * qs + id] = (float)(fq - wq(q));
         } else {
             // scaled deviation rounded to FP32 first, then to the 16-bit format
             const float v = (float)((fq - wq(q)) * (double)sc.s[q]);
-            CD::st(F, (long long)q * n + id, v, 1.0f);
+            CD::st(F, (long long)q * qs + id, v, 1.0f);
         }
     }
     C* M = (C*)mac;
@@ -333,27 +333,28 @@ __global__ __launch_bounds__(256) void k_init_equilibrium(void* f, void* mac, lo
     M[3 * n + id] = (C)uu[2];
 }
 
-hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, void* mac, long long n, int nx, int ny, int nz,
-                                   double rho0, const double u0[3], double amp, uint64_t seed, const float* scale)
+hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, long long qs, void* mac, long long n, int nx,
+                                   int ny, int nz, double rho0, const double u0[3], double amp, uint64_t seed,
+                                   const float* scale)
 {
     InitScale sc;
     for (int q = 0; q < NQ; ++q) sc.s[q] = scale ? scale[q] : 1.0f;
     const unsigned nb = (unsigned)((n + 255) / 256);
     switch (store) {
     case Store::F64:
-        hipLaunchKernelGGL(k_init_equilibrium<Store::F64>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0, u0[0],
+        hipLaunchKernelGGL(k_init_equilibrium<Store::F64>, dim3(nb), dim3(256), 0, st, f, qs, mac, n, nx, ny, nz, rho0, u0[0],
                            u0[1], u0[2], amp, seed, sc);
         break;
     case Store::F32:
-        hipLaunchKernelGGL(k_init_equilibrium<Store::F32>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0, u0[0],
+        hipLaunchKernelGGL(k_init_equilibrium<Store::F32>, dim3(nb), dim3(256), 0, st, f, qs, mac, n, nx, ny, nz, rho0, u0[0],
                            u0[1], u0[2], amp, seed, sc);
         break;
     case Store::BF16:
-        hipLaunchKernelGGL(k_init_equilibrium<Store::BF16>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0,
+        hipLaunchKernelGGL(k_init_equilibrium<Store::BF16>, dim3(nb), dim3(256), 0, st, f, qs, mac, n, nx, ny, nz, rho0,
                            u0[0], u0[1], u0[2], amp, seed, sc);
         break;
     case Store::FP16:
-        hipLaunchKernelGGL(k_init_equilibrium<Store::FP16>, dim3(nb), dim3(256), 0, st, f, mac, n, nx, ny, nz, rho0,
+        hipLaunchKernelGGL(k_init_equilibrium<Store::FP16>, dim3(nb), dim3(256), 0, st, f, qs, mac, n, nx, ny, nz, rho0,
                            u0[0], u0[1], u0[2], amp, seed, sc);
         break;
     }

@@ -80,6 +80,7 @@ struct StepArgs {
     const double* rho_modify;
     int nx, ny, nz;
     long long n;
+    long long qs;  // population stride (nodes)
     double tau, dt, cs;
     int les;
     float scale[NQ];
@@ -106,6 +107,7 @@ inline StepArgs<ST> make_step_args(const StepLaunch& L)
     a.ny = L.g.ny;
     a.nz = L.g.nz;
     a.n = L.g.n;
+    a.qs = L.g.qs;
     a.tau = L.tau;
     a.dt = L.dt;
     a.cs = L.cs_smag;
@@ -269,7 +271,7 @@ __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long 
                                    const int* xo, const int* yo, long long plane)
 {
     const long long N = a.n;
-    auto at = [&](int q) { return Codec<ST>::ld(a.fo + (long long)q * N, node, a.inv_scale[q]); };
+    auto at = [&](int q) { return Codec<ST>::ld(a.fo + (long long)q * a.qs, node, a.inv_scale[q]); };
     if (k == 0) {
         F[5] = at(6); F[11] = at(14); F[12] = at(13); F[15] = at(18); F[16] = at(17);
     }
@@ -279,7 +281,7 @@ __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long 
     if (k <= 1) {
         const int nb = 2 * a.ny;
         const int ib = k * a.ny + j;
-        auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * N, a.inv_scale[q]}; };
+        auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
         auto W = [&](int which) { return a.t.bw + ((long long)which * nb + ib) * 7; };
         const int f3 = a.t.bflag[B3 * nb + ib], f4 = a.t.bflag[B4 * nb + ib];
         const int f15 = a.t.bflag[B15 * nb + ib], f16 = a.t.bflag[B16 * nb + ib];

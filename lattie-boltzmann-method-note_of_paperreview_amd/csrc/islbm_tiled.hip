@@ -125,13 +125,22 @@ TiledGeom tiled_geom(int nx, int ny, int nz, int ty, int resident)
     t.ntx = (nx + TX - 1) / TX;
     t.nty = (ny + ty - 1) / ty;
     t.ntiles = t.ntx * t.nty;
-    // enough z chunks to fill the resident workgroup slots, of at least 16 levels (each
-    // chunk re-reads a 6-level xi halo)
-    int want = (resident + t.ntiles - 1) / t.ntiles;
-    int maxc = nz / 16 > 0 ? nz / 16 : 1;
-    t.nchunks = want < 1 ? 1 : (want > maxc ? maxc : want);
-    t.kc = (nz + t.nchunks - 1) / t.nchunks;
-    t.nchunks = (nz + t.kc - 1) / t.kc;
+    // z chunks of at least 16 levels (each chunk re-reads a 6-level xi halo), as many as
+    // minimise the makespan: rounds of `resident` workgroups times the steps of a chunk
+    // (the fewest chunks among equal makespans)
+    const int maxc = nz / 16 > 0 ? nz / 16 : 1;
+    long long best = -1;
+    for (int c = 1; c <= maxc; ++c) {
+        const int kc = (nz + c - 1) / c;
+        const int cc = (nz + kc - 1) / kc;
+        const long long rounds = ((long long)t.ntiles * cc + resident - 1) / resident;
+        const long long cost = rounds * (kc + 6);
+        if (best < 0 || cost < best) {
+            best = cost;
+            t.kc = kc;
+            t.nchunks = cc;
+        }
+    }
     t.nwork = t.ntiles * t.nchunks;
     t.nblocks = ((t.nwork + 7) / 8) * 8;
     return t;
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long long plane = (long long)nx * ny;
     const long long N = a.n;
-    const unsigned nbq = SZ * (unsigned)N;  // bytes of one population array; 19 of them < 2^32 (tiled_supported)
+    const unsigned nbq = SZ * (unsigned)a.qs;  // bytes from one population array to the next; 19 of them < 2^32
     const unsigned xi_bytes = 4u * (unsigned)nz * (unsigned)ny * (NXI * 7);
     const int lane = threadIdx.x % 64;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
@@ -853,7 +862,7 @@ bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
     // buffer resources address a population buffer (19 arrays), a field or the xi table
     // with 32-bit byte offsets
     const long long sz = store == Store::F32 ? 4 : 2;
-    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && NQ * g.n * sz < (1LL << 32) &&
+    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && NQ * g.qs * sz < (1LL << 32) &&
            4 * g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
 }
 

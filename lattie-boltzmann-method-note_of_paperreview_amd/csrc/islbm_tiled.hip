@@ -1,30 +1,33 @@
 // islbm_tiled.hip -- column-marching, LDS-staged, sum-factorised stream-collide kernel for
 // FP32 and 16-bit population storage (FP32 arithmetic, deviation form f - w_q).
 //
-// Work decomposition.  A workgroup owns a 64 (x) by 2 (y) tile of node columns and
-// marches up a chunk of z levels.  It has six waves in two roles:
-//   ring waves (4): two direction groups x two y rows; a lane keeps a ring of its group's
-//       8 directions x 7 levels in registers
+// Work decomposition.  A workgroup owns a 64 (x) by TY (y) tile of node columns (TY = 4 with
+// 16-byte staging, 2 with 4-byte staging) and marches up a chunk of z levels.  It has 3 TY
+// waves in three roles, one wave of each role per y row:
+//   ring waves, group A and group B: a lane keeps a ring of its group's 8 directions x 7
+//       levels in registers
 //           group A: 5 6 7 8 9 10 11 12        group B: 3 4 13 14 15 16 17 18
-//   collision waves (2): one per y row; they take the 16 interpolated populations and
-//       the staged rows of 0, 1, 2 from LDS, apply the walls, collide (collide_pairs,
-//       evolution.h:597-679) and store; they issue no global load.
+//   collision waves: they take the 19 populations of a target from LDS, apply the walls,
+//       collide (collide_pairs, evolution.h:597-679) and store; they issue no global load.
 // The roles run separate loops with one workgroup barrier per step, so each role's
 // registers are allocated for that role alone and the collision of one target overlaps
 // the ring work of the next.
 //
 // Per z level the ring waves stage the 14 populations with horizontal neighbours into LDS
-// (double-buffered, with a 3-node halo where the direction needs one: 7-10 in x and y,
-// 3, 4, 15-18 in y, 11-14 in x; 5 and 6 at the nodes only), run the
-// separable x and y Lagrange filters from LDS (group A runs the y pass of 7-10 for its own
-// row in a wave-private LDS area) and append the filtered values to the ring.  Once the
-// ring covers a target level's xi stencil, each ring wave applies the xi weights of the
-// target node to its 8 directions (the last step of the interpolated streaming of
-// interpolationHillISLBM.h) and hands the results to the collision waves through LDS.
+// (three level buffers, loads issued two steps ahead, with a 3-node halo where the
+// direction needs one: 7-10 in x and y, 3, 4, 15-18 in y, 11-14 in x; 5 and 6 at the nodes
+// only), run the separable x and y Lagrange filters from LDS (group A runs the y pass of
+// 7-10 for its own row in a wave-private LDS area) and append the filtered values to the
+// ring.  Once the ring covers a target level's xi stencil, each ring wave applies the xi
+// weights of the target node to its 8 directions (the last step of the interpolated
+// streaming of interpolationHillISLBM.h) and hands the results to the collision waves
+// through LDS; group B also hands over directions 0, 1 and 2 (no z dependence: the node's
+// 0 and the x filters of 1 and 2 over the target level's staged rows).
 //
-// Staging uses buffer loads straight into LDS (buffer_load_dword/ushort ... lds), one
-// 64-lane row per load, so no registers hold the prefetch; 16-bit storage lands as one
-// zero-extended dword per lane and is decoded in place by the wave that loaded it.
+// Staging uses buffer loads straight into LDS, so no registers hold the prefetch: with
+// 16-byte staging one dwordx4 load of 64 lanes fills 64 granules of a level; with 4-byte
+// staging one 64-lane row per load, 16-bit storage landing as one zero-extended dword per
+// lane, decoded in place by the wave that loaded it.
 // Blocks are mapped to tiles so that each XCD (blocks dispatched round-robin over 8 XCDs)
 // works on a contiguous slab of tiles, keeping the y-halo rows of neighbouring tiles in
 // one L2.  The uniform mesh in x and y makes the x and y weights identical for every
@@ -63,7 +66,7 @@ struct Lay {
     static constexpr int NTH = 64 * (NRW + NCW);
     static constexpr int HY = TY + 6;
     static constexpr int Y_SZ = HY * TX;       // y halo only
-    static constexpr int HO_SZ = 16 * TX * TY;  // hand-over: directions 3-18 x TX TY columns
+    static constexpr int HO_SZ = 19 * TX * TY;  // hand-over: directions 0-18 x TX TY columns
     static constexpr int HXW = W ? TX + 2 * GV : HX;  // x-halo row stride: whole granules from i0 - GV
     static constexpr int XO = W ? GV - 3 : 0;           // column of x = i0 - 3
     static constexpr int XY_SZ = HY * HXW;
@@ -81,7 +84,8 @@ struct Lay {
     static constexpr int OFF_HO = OFF_YP + TY * 4 * HXW;
     // per-target tables staged two steps ahead, NBUF slots (slot = hand-over step % NBUF)
     static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, NBUF x TY rows x 64
-    // directions 0, 1, 2 of a target level, 2 steps x TY rows: 0 at its node, 1 and 2 with x halo
+    // rows of directions 0, 1, 2 of a target level, NBUF slots x TY rows: 0 at its nodes, 1
+    // and 2 with the x halo (read by group B)
     static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
     static constexpr int OFF_P012 = OFF_XIB + NBUF * TY * 64;
     // transposed result stores: an area per collision wave, the 19 codes of its 64 nodes as
@@ -95,7 +99,7 @@ struct Lay {
                          OFF_P012 % 4 == 0 && P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
 };
-// 96 KB (4-byte staging) or 151 KB (16-byte staging): one workgroup per CU
+// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32) or 126 KB (BF16): one workgroup per CU
 static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024 &&
                   Lay<true, 2, true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
@@ -323,6 +327,47 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     const int iw = i < nx ? i : i % nx;
     const auto rfo = rsrc_of(a.fo, NQ * nbq);
     auto qoff = [&](int q) { return (unsigned)q * nbq; };
+    // directions 0, 1, 2 of a target (no z dependence) from its staged rows P: the node's own
+    // 0 and the 7 x taps of 1 and 2 (weights w1, w2), in FP32 deviation form
+    auto reduce012 = [&](const float* P, const float* w1, const float* w2, float& f0, float& f1, float& f2) {
+        const float isc0 = a.inv_scale[0], isc1 = a.inv_scale[1], isc2 = a.inv_scale[2];
+        if constexpr (W && k16) {
+            // 16-bit codes: direction 0 at the nodes, 1 and 2 with the x halo
+            const unsigned short* P16 = (const unsigned short*)P;
+            const unsigned short* Q1 = P16 + TX + XO + tx;
+            const unsigned short* Q2 = P16 + TX + HXW + XO + tx;
+            f0 = dec16<ST>(P16[tx]) * isc0;
+            float s1 = dec16<ST>(Q1[0]) * w1[0], s2 = dec16<ST>(Q2[0]) * w2[0];
+#pragma unroll
+            for (int l = 1; l < 7; ++l) {
+                s1 = s1 + dec16<ST>(Q1[l]) * w1[l];
+                s2 = s2 + dec16<ST>(Q2[l]) * w2[l];
+            }
+            f1 = s1 * isc1;
+            f2 = s2 * isc2;
+            return;
+        }
+        // 4-byte staging: FP32 as stored, 16-bit codes zero-extended to a dword
+        auto dec = [&](float v, float isc) -> float {
+            if constexpr (k16) {
+                const raw r = __builtin_bit_cast(raw, (unsigned short)(__float_as_uint(v) & 0xFFFFu));
+                return CD::ld(&r, 0, isc);
+            } else {
+                return v;
+            }
+        };
+        f0 = dec(P[tx], isc0);
+        const float* P1 = P + TX + XO + tx;
+        const float* P2 = P + TX + HXW + XO + tx;
+        float s1 = dec(P1[0], isc1) * w1[0], s2 = dec(P2[0], isc2) * w2[0];
+#pragma unroll
+        for (int l = 1; l < 7; ++l) {
+            s1 = s1 + dec(P1[l], isc1) * w1[l];
+            s2 = s2 + dec(P2[l], isc2) * w2[l];
+        }
+        f1 = s1;
+        f2 = s2;
+    };
 
     if (wave < NRW) {
         // ------------------------------------------------------------------ ring waves
@@ -445,9 +490,8 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
 #pragma unroll
                 for (int m = 0; m < 8; ++m) h[d][m] = 0.f;
             // Every vector-memory instruction of a ring wave is a staging load into LDS, issued
-            // two steps before it is read.  In step s, group B stages the xi weights of the
-            // target handed over in step s+2 and the rows of 0-2 of the target handed over in
-            // step s+1 (the collision waves read them in step s+2), then every ring wave issues
+            // two steps before it is read.  In step s, group B stages the xi weights and the
+            // rows of 0-2 of the target handed over in step s+2, then every ring wave issues
             // its share of the level two above the one it appends (at the top of the column,
             // level L1 again into the free buffer).  Loads complete in order, so a step waits
             // until only the loads of the step before are outstanding (vmcnt = their count).
@@ -493,8 +537,8 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                         rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + ty) * 64), 4,
                         (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
             };
-            // rows of directions 0, 1, 2 of target tk for the collision waves, staged by the
-            // group B wave of each row (no z dependence)
+            // rows of directions 0, 1, 2 of target tk, staged by the group B wave of each row
+            // for its own hand-over (no z dependence)
             auto fetch012 = [&](int tk, int par) {
                 if constexpr (G == 1 && W) {
                     // one 16-byte load: granules 0-15 direction 0, 16-33 direction 1 and 34-51
@@ -522,16 +566,17 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                 h2 = h2 >= 0 ? h2 : k0;
             };
             {
-                // the two steps before step 0: level L0 and the xi weights of step 0, then the
-                // xi weights of step 1, the rows of 0-2 of step 0 and level L0 + 1
+                // the two steps before step 0: level L0, the xi weights and rows of 0-2 of
+                // step 0, then those of step 1 and level L0 + 1
                 Sched n = sc;
                 bool a_;
                 int h0, h1;
                 sched_step(n, k1, nz, a_, h0);
                 sched_step(n, k1, nz, a_, h1);
                 fetch_xi(h0 >= 0 ? h0 : k0, 0);
-                fetch_xi(h1 >= 0 ? h1 : k0, 1);
                 fetch012(h0 >= 0 ? h0 : k0, 0);
+                fetch_xi(h1 >= 0 ? h1 : k0, 1);
+                fetch012(h1 >= 0 ? h1 : k0, 1);
             }
             fetch(min(L0 + 1, L1), 1);
             int outstanding = ntab + nbatch;  // loads issued in the step before
@@ -551,7 +596,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                     lookahead(sc, h1, h2);
                     const int s1 = (s + 1) % NBUF, s2 = (s + 2) % NBUF;
                     fetch_xi(h2, s2);   // read by the ring waves in step s+2
-                    fetch012(h1, s1);   // read by the collision waves in step s+2
+                    fetch012(h2, s2);   // read by the group B waves in step s+2
                 }
                 if (adv) {
                     // the buffer after the next holds level x-1, filtered in the last append step
@@ -647,7 +692,15 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                         float sum = win(d, 0) * wz[0];
 #pragma unroll
                         for (int m = 1; m < 7; ++m) sum = sum + win(d, m) * wz[m];
-                        HO[(q - 3) * (TX * TY) + col] = sum;
+                        HO[q * (TX * TY) + col] = sum;
+                    }
+                    if constexpr (G == 1) {
+                        // directions 0, 1, 2 of the target from its rows, staged two steps ahead
+                        float f0, f1, f2;
+                        reduce012(lds + OFF_P012 + ((s % NBUF) * TY + ty) * P_ROW, wx[0], wx[1], f0, f1, f2);
+                        HO[col] = f0;
+                        HO[TX * TY + col] = f1;
+                        HO[2 * TX * TY + col] = f2;
                     }
                 };
                 // after step s the window is that of parity s % 2: an appending step writes
@@ -683,9 +736,8 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                 if (!step(std::integral_constant<int, 0>{})) break;
                 if (!step(std::integral_constant<int, 1>{})) break;
             }
-            // the final step, in which the collision waves take the last target: its rows of
-            // 0-2 must be in LDS before the barrier, and no staging load may still write LDS
-            // when the workgroup retires
+            // the final step, in which the collision waves take the last target: no staging
+            // load may still write LDS when the workgroup retires
             __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
             lds_barrier();
         };
@@ -705,13 +757,6 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         const unsigned own_b = SZ * own;
         const unsigned own4 = 4u * own;
         const unsigned row = (unsigned)(jw * nx);
-        float wx0[7], wx2[7];
-#pragma unroll
-        for (int l = 0; l < 7; ++l) {
-            wx0[l] = a.t.xw0[l];
-            wx2[l] = a.t.xw2[l];
-        }
-        const float isc0 = a.inv_scale[0], isc1 = a.inv_scale[1], isc2 = a.inv_scale[2];
         // Scalars and tables that no kernel writes during the step are read through the
         // constant address space (scalar loads): a vector load here would be waited for
         // together with the next target's loads issued just before it.
@@ -728,45 +773,6 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         // columns of a partial tile store their owner's values
         const unsigned st16 = (unsigned)(lane / 8) * nbq + SZ * (unsigned)((i0 + 8 * (lane % 8)) % nx);
         const unsigned st16m = (unsigned)(lane / 16) * fbytes + 4u * (unsigned)((i0 + 4 * (lane % 16)) % nx);
-        // directions 0, 1, 2 of the target (no z dependence), staged in LDS by the ring
-        // waves: the node's own 0 and the 7 x taps of 1 and 2, reduced to F0, F1, F2
-        auto dec = [&](float v, float isc) -> float {
-            if constexpr (k16) {
-                const raw r = __builtin_bit_cast(raw, (unsigned short)(__float_as_uint(v) & 0xFFFFu));
-                return CD::ld(&r, 0, isc);
-            } else {
-                return v;
-            }
-        };
-        auto reduce012 = [&](float (&F)[NQ], const float* P) {
-            if constexpr (W && k16) {
-                // 16-bit codes: direction 0 at the nodes, 1 and 2 with the x halo
-                const unsigned short* P16 = (const unsigned short*)P;
-                const unsigned short* Q1 = P16 + TX + XO + tx;
-                const unsigned short* Q2 = P16 + TX + HXW + XO + tx;
-                F[0] = dec16<ST>(P16[tx]) * isc0;
-                float s1 = dec16<ST>(Q1[0]) * wx0[0], s2 = dec16<ST>(Q2[0]) * wx2[0];
-#pragma unroll
-                for (int l = 1; l < 7; ++l) {
-                    s1 = s1 + dec16<ST>(Q1[l]) * wx0[l];
-                    s2 = s2 + dec16<ST>(Q2[l]) * wx2[l];
-                }
-                F[1] = s1 * isc1;
-                F[2] = s2 * isc2;
-                return;
-            }
-            F[0] = dec(P[tx], isc0);
-            const float* P1 = P + TX + XO + tx;
-            const float* P2 = P + TX + HXW + XO + tx;
-            float s1 = dec(P1[0], isc1) * wx0[0], s2 = dec(P2[0], isc2) * wx2[0];
-#pragma unroll
-            for (int l = 1; l < 7; ++l) {
-                s1 = s1 + dec(P1[l], isc1) * wx0[l];
-                s2 = s2 + dec(P2[l], isc2) * wx2[l];
-            }
-            F[1] = s1;
-            F[2] = s2;
-        };
         // walls, collision and stores of target level tk.  Lanes outside the grid (partial
         // tiles) hold the wrapped column or row, whose values they compute bit for bit like
         // its owner, so their stores duplicate the owner's: the stores need no lane mask,
@@ -843,12 +849,11 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         // inputs all come through LDS.
         for (;;) {
             const bool more = sched_step(sc, k1, nz, adv, ho);
-            lds_barrier();  // hand-over of pend (step s-1) and its rows of 0-2 in LDS
+            lds_barrier();  // hand-over of pend (step s-1) in LDS
             float F[NQ];
             const float* HO = lds + OFF_HO + ((s - 1) & 1) * HO_SZ;
 #pragma unroll
-            for (int q = 3; q < NQ; ++q) F[q] = HO[(q - 3) * (TX * TY) + col];
-            reduce012(F, lds + OFF_P012 + (((s - 1) % NBUF) * TY + ty) * P_ROW);
+            for (int q = 0; q < NQ; ++q) F[q] = HO[q * (TX * TY) + col];
             collide_store(F, pend);
             if (!more) break;  // final step: the ring has appended and handed over everything
             pend = ho;

@@ -48,10 +48,8 @@ struct Codec<Store::BF16> {
     }
     __device__ static void st(raw* p, long long i, C v, float sc)
     {
-        // round to nearest even
-        const uint32_t b = __float_as_uint(v * sc);
-        const uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
-        p[i] = (uint16_t)(r >> 16);
+        // round to nearest even (v_cvt_pk_bf16_f32)
+        p[i] = __builtin_bit_cast(uint16_t, (__bf16)(v * sc));
     }
 };
 

@@ -55,9 +55,7 @@ constexpr int HX = TX + 6;
 // A staged level holds
 //   XY: 7-10 [4][HY][HXW]   X: 11-14 [4][TY][HXW]   Z: 5, 6 [2][TY][TX]   Y: 3 4 15-18 [6][HY][TX]
 // (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).
-// VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16); TS: transposed result
-// stores (BF16 with 16-byte staging)
-template <bool W, int VB = 4, bool TS = false>
+template <bool W, int VB = 4>  // VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16)
 struct Lay {
     static constexpr int GV = 16 / VB;  // values per 16-byte granule
     static constexpr int TY = W ? 4 : 2;
@@ -88,20 +86,16 @@ struct Lay {
     // and 2 with the x halo (read by group B)
     static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
     static constexpr int OFF_P012 = OFF_XIB + NBUF * TY * 64;
-    // transposed result stores: an area per collision wave, the 19 codes of its 64 nodes as
-    // 19 rows of 64, then rho u v w as 4 rows of 64 floats
-    static constexpr int OFF_ST = OFF_P012 + NBUF * TY * P_ROW;
-    static constexpr int ST_SZ = (W && VB == 2 && TS) ? 19 * TX / 2 + 4 * TX : 0;
-    static constexpr int LDS_FLOATS = OFF_ST + TY * ST_SZ;
+    static constexpr int LDS_FLOATS = OFF_P012 + NBUF * TY * P_ROW;
     // workgroups resident at once on 256 CUs: the LDS-DMA staging runs at one per CU
     static constexpr int RESIDENT = W ? 256 : 512;
     static_assert(!W || (OFF_X % GV == 0 && OFF_Z % GV == 0 && OFF_Y % GV == 0 && STAGE % 4 == 0 &&
                          OFF_P012 % 4 == 0 && P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
 };
-// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32) or 126 KB (BF16): one workgroup per CU
+// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32) or 112 KB (BF16, FP16): one workgroup per CU
 static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024 &&
-                  Lay<true, 2, true>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
+                  Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
@@ -291,10 +285,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     using raw = typename CD::raw;
     using IO = BufIO<ST>;
     constexpr unsigned SZ = sizeof(raw);
-    // BF16 results leave through 16-byte stores transposed in LDS (measured at 128 x 256 x 128:
-    // faster for BF16, slower for FP16 and FP32, which store value by value)
-    constexpr bool TS = W && ST == Store::BF16;
-    using LY = Lay<W, W ? (int)SZ : 4, TS>;
+    using LY = Lay<W, W ? (int)SZ : 4>;
     constexpr bool k16 = (ST != Store::F32);
     constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
@@ -767,12 +758,6 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         const auto rfn = rsrc_of(a.fn, NQ * nbq);
         // rho, u, v, w are consecutive arrays of one allocation (checked at launch)
         const auto rmac = rsrc_of(a.rho, 4 * fbytes);
-        // transposed result stores: lane l writes granule l % 8 of population row 8 n + l / 8
-        // of store n (columns i0 + 8 (l % 8) .. + 7, wrapped as a whole since nx % 8 == 0) and
-        // granule l % 16 of macro row l / 16 (columns i0 + 4 (l % 16) .. + 3); the duplicate
-        // columns of a partial tile store their owner's values
-        const unsigned st16 = (unsigned)(lane / 8) * nbq + SZ * (unsigned)((i0 + 8 * (lane % 8)) % nx);
-        const unsigned st16m = (unsigned)(lane / 16) * fbytes + 4u * (unsigned)((i0 + 4 * (lane % 16)) % nx);
         // walls, collision and stores of target level tk.  Lanes outside the grid (partial
         // tiles) hold the wrapped column or row, whose values they compute bit for bit like
         // its owner, so their stores duplicate the owner's: the stores need no lane mask,
@@ -795,36 +780,6 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                                 tau_eff);
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
-            if constexpr (TS) {
-                // this lane's column of every result into the wave's transposed area, then
-                // 16-byte granules back out: the 19 directions (8 codes of one direction per
-                // granule) in 3 stores, the last with 24 lanes, and rho u v w in one
-                float* T = lds + LY::OFF_ST + ty * LY::ST_SZ;
-                unsigned short* T16 = (unsigned short*)T;
-                float* M = T + NQ * TX / 2;
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                    raw c;
-                    CD::st(&c, 0, F[q], a.scale[q]);
-                    T16[q * TX + tx] = __builtin_bit_cast(unsigned short, c);
-                }
-                M[tx] = rho;
-                M[TX + tx] = ux;
-                M[2 * TX + tx] = uy;
-                M[3 * TX + tx] = uz;
-                __builtin_amdgcn_wave_barrier();
-                using v4u = unsigned __attribute__((ext_vector_type(4)));
-#pragma unroll
-                for (int n = 0; n < 3; ++n) {
-                    const v4u v = *(const v4u*)(T + 4 * (64 * n + lane));
-                    if (n < 2 || lane < 24)
-                        __builtin_amdgcn_raw_buffer_store_b128(v, rfn, (int)st16, (int)(8u * n * nbq + lv + SZ * row),
-                                                               kStoreAux);
-                }
-                const v4u m = *(const v4u*)(M + 4 * lane);
-                __builtin_amdgcn_raw_buffer_store_b128(m, rmac, (int)st16m, (int)(lf + 4u * row), kStoreAux);
-                return;
-            }
 #pragma unroll
             for (int q = 0; q < NQ; ++q) IO::st(rfn, own_b, qoff(q) + lv, F[q], a.scale[q]);
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rmac, (int)own4, (int)lf, kStoreAux);

@@ -133,7 +133,9 @@ def test_reduced_precision_against_fp64_oracle(pkg, po, prec, tol):
     compare(o, s, tol, prec)
 
 
-def test_reference_named_entry_points_match_native(pkg, po):
+# 64 x 128 x 64: FP64 population arrays of exactly 4 MiB, which the solver places 5 MiB apart
+@pytest.mark.parametrize("grid", [(16, 32, 32), (64, 128, 64)])
+def test_reference_named_entry_points_match_native(pkg, po, grid):
     """Launch_CollisionStreaming on the reference's padded device layout (jp = 1): four
     ping-pong calls from a seeded state equal four oracle steps, and every ghost and
     periodic-duplicate layer of f_new holds the periodic image."""
@@ -144,7 +146,7 @@ def test_reference_named_entry_points_match_native(pkg, po):
     hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
     hip.hipFree.argtypes = [ct.c_void_p]
     H2D, D2H = 1, 2
-    nx, ny, nz = 16, 32, 32
+    nx, ny, nz = grid
     NX6, NY6, NZ6 = nx + 7, ny + 7, nz + 6
     p = pkg.default_params(nx=nx, ny=ny, nz=nz)
     assert L.islbm_ref_init(ct.byref(p)) == 0, L.islbm_last_error()

@@ -635,19 +635,31 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                         return sum;
                     };
                     if constexpr (G == 0) {
-                        // y pass of 7-10 on this wave's row, all HX columns, into its own LDS
-                        // area (read back by this wave only: no barrier needed)
+                        // y pass of 7-10 on this wave's row into its own LDS area (read back
+                        // by this wave only: no barrier needed), over the 70 columns the x pass
+                        // reads (from x = i0 - 3): 64 per direction, then the last 6 of all
+                        // four directions in one pass of 24 lanes
                         float* YP = lds + OFF_YP + ty * 4 * HXW;
 #pragma unroll
                         for (int d = 0; d < 4; ++d) {
                             const float* wyy = wy[ysel(7 + d)];
-                            for (int c = lane; c < HXW; c += 64) {
-                                const int o = OFF_XY + d * XY_SZ + ty * HXW + c;
-                                float sum = sv(o) * wyy[0];
+                            const int c = XO + lane;
+                            const int o = OFF_XY + d * XY_SZ + ty * HXW + c;
+                            float sum = sv(o) * wyy[0];
 #pragma unroll
-                                for (int n = 1; n < 7; ++n) sum = sum + sv(o + n * HXW) * wyy[n];
-                                YP[d * HXW + c] = sum;
-                            }
+                            for (int n = 1; n < 7; ++n) sum = sum + sv(o + n * HXW) * wyy[n];
+                            YP[d * HXW + c] = sum;
+                        }
+                        static_assert(ysel(7) == ysel(8) && ysel(9) == ysel(10), "7, 8 and 9, 10 share y weights");
+                        if (lane < 24) {
+                            const int d = lane / 6, c = XO + 64 + lane % 6;
+                            const bool first = d < 2;
+                            const int o = OFF_XY + d * XY_SZ + ty * HXW + c;
+                            float sum = sv(o) * (first ? wy[ysel(7)][0] : wy[ysel(9)][0]);
+#pragma unroll
+                            for (int n = 1; n < 7; ++n)
+                                sum = sum + sv(o + n * HXW) * (first ? wy[ysel(7)][n] : wy[ysel(9)][n]);
+                            YP[d * HXW + c] = sum;
                         }
                         nv[0] = scl(sv(OFF_Z + ty * TX + tx), 5);         // 5 at the node (no filter)
                         nv[1] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);  // 6

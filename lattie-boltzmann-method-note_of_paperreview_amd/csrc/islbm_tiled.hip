@@ -21,8 +21,8 @@
 // ring.  Once the ring covers a target level's xi stencil, each ring wave applies the xi
 // weights of the target node to its 8 directions (the last step of the interpolated
 // streaming of interpolationHillISLBM.h) and hands the results to the collision waves
-// through LDS; group B also hands over directions 0, 1 and 2 (no z dependence: the node's
-// 0 and the x filters of 1 and 2 over the target level's staged rows).
+// through LDS; group A also hands over directions 0, 1 and 2 (no z dependence: the node's
+// 0 and the x filters of 1 and 2 over the target level's rows, which group B stages).
 //
 // Staging uses buffer loads straight into LDS, so no registers hold the prefetch: with
 // 16-byte staging one dwordx4 load of 64 lanes fills 64 granules of a level; with 4-byte
@@ -83,7 +83,7 @@ struct Lay {
     // per-target tables staged two steps ahead, NBUF slots (slot = hand-over step % NBUF)
     static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, NBUF x TY rows x 64
     // rows of directions 0, 1, 2 of a target level, NBUF slots x TY rows: 0 at its nodes, 1
-    // and 2 with the x halo (read by group B)
+    // and 2 with the x halo (staged by group B, read by group A)
     static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
     static constexpr int OFF_P012 = OFF_XIB + NBUF * TY * 64;
     static constexpr int LDS_FLOATS = OFF_P012 + NBUF * TY * P_ROW;
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                         (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
             };
             // rows of directions 0, 1, 2 of target tk, staged by the group B wave of each row
-            // for its own hand-over (no z dependence)
+            // for the group A hand-over (no z dependence)
             auto fetch012 = [&](int tk, int par) {
                 if constexpr (G == 1 && W) {
                     // one 16-byte load: granules 0-15 direction 0, 16-33 direction 1 and 34-51
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                     lookahead(sc, h1, h2);
                     const int s1 = (s + 1) % NBUF, s2 = (s + 2) % NBUF;
                     fetch_xi(h2, s2);   // read by the ring waves in step s+2
-                    fetch012(h2, s2);   // read by the group B waves in step s+2
+                    fetch012(h2, s2);   // read by the group A waves in step s+2
                 }
                 if (adv) {
                     // the buffer after the next holds level x-1, filtered in the last append step
@@ -697,8 +697,9 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                         for (int m = 1; m < 7; ++m) sum = sum + win(d, m) * wz[m];
                         HO[q * (TX * TY) + col] = sum;
                     }
-                    if constexpr (G == 1) {
-                        // directions 0, 1, 2 of the target from its rows, staged two steps ahead
+                    if constexpr (G == 0) {
+                        // directions 0, 1, 2 of the target from its rows, staged two steps ahead by
+                        // group B (its wait for them precedes the barrier of this step)
                         float f0, f1, f2;
                         reduce012(lds + OFF_P012 + ((s % NBUF) * TY + ty) * P_ROW, wx[0], wx[1], f0, f1, f2);
                         HO[col] = f0;

@@ -7,6 +7,7 @@
 //   ring waves, group A and group B: a lane keeps a ring of its group's 8 directions x 7
 //       levels in registers
 //           group A: 5 6 7 8 9 10 11 12        group B: 3 4 13 14 15 16 17 18
+//       (FP16 storage: 5 and 13 swapped, see gq)
 //   collision waves: they take the 19 populations of a target from LDS, apply the walls,
 //       collide (collide_pairs, evolution.h:597-679) and store; they issue no global load.
 // The roles run separate loops with one workgroup barrier per step, so each role's
@@ -102,8 +103,14 @@ __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n 
 
 // staged direction of each y-halo LDS region
 __host__ __device__ constexpr int qy(int d) { return d < 2 ? 3 + d : 13 + d; }  // 3 4 15 16 17 18
-// the 8 directions of ring group G
-__host__ __device__ constexpr int gq(int G, int d) { return G == 0 ? 5 + d : (d < 2 ? 3 + d : 11 + d); }
+// the 8 directions of ring group G.  SW: directions 5 and 13 swapped between the groups
+// (group A: 6 13 7-12, group B: 3 4 5 14 15-18), which balances the groups' per-step work
+// where 16-bit decoding folds into the filters' FMAs (FP16 storage)
+__host__ __device__ constexpr int gq(int G, int d, bool SW)
+{
+    return G == 0 ? (SW && d == 0 ? 6 : SW && d == 1 ? 13 : 5 + d)
+                  : (d < 2 ? 3 + d : SW && d == 2 ? 5 : 11 + d);
+}
 __host__ __device__ constexpr int xsel(int q) { return cx(q) > 0 ? 0 : 1; }  // 0: X0 (x - dt), 1: X2
 __host__ __device__ constexpr int ysel(int q) { return cy(q) > 0 ? 0 : 1; }
 // xi family of each interpolated direction (initialization.h:235-242, evolution.h:457-475)
@@ -287,6 +294,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     constexpr unsigned SZ = sizeof(raw);
     using LY = Lay<W, W ? (int)SZ : 4>;
     constexpr bool k16 = (ST != Store::F32);
+    constexpr bool SW = (ST == Store::FP16);  // ring group composition (gq)
     constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
     constexpr int OFF_YP = LY::OFF_YP, OFF_HO = LY::OFF_HO, OFF_XIB = LY::OFF_XIB, OFF_P012 = LY::OFF_P012;
@@ -661,8 +669,13 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                                 sum = sum + sv(o + n * HXW) * (first ? wy[ysel(7)][n] : wy[ysel(9)][n]);
                             YP[d * HXW + c] = sum;
                         }
-                        nv[0] = scl(sv(OFF_Z + ty * TX + tx), 5);         // 5 at the node (no filter)
-                        nv[1] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);  // 6
+                        if constexpr (SW) {
+                            nv[0] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);                           // 6 at the node
+                            nv[1] = scl(xf(OFF_X + 2 * X_SZ + ty * HXW + XO + tx, wx[xsel(13)]), 13);  // 13: x filter
+                        } else {
+                            nv[0] = scl(sv(OFF_Z + ty * TX + tx), 5);         // 5 at the node (no filter)
+                            nv[1] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);  // 6
+                        }
 #pragma unroll
                         for (int d = 0; d < 4; ++d)  // 7-10: x pass over the y-passed row
                             nv[2 + d] = scl(xfp(YP + d * HXW + XO + tx, wx[xsel(7 + d)]), 7 + d);
@@ -673,9 +686,14 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
 #pragma unroll
                         for (int d = 0; d < 2; ++d)  // 3, 4: y filter
                             nv[d] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(3 + d)]), 3 + d);
+                        if constexpr (SW) {
+                            nv[2] = scl(sv(OFF_Z + ty * TX + tx), 5);                                  // 5 at the node
+                            nv[3] = scl(xf(OFF_X + 3 * X_SZ + ty * HXW + XO + tx, wx[xsel(14)]), 14);  // 14: x filter
+                        } else {
 #pragma unroll
-                        for (int d = 2; d < 4; ++d)  // 13, 14: x filter
-                            nv[d] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
+                            for (int d = 2; d < 4; ++d)  // 13, 14: x filter
+                                nv[d] = scl(xf(OFF_X + d * X_SZ + ty * HXW + XO + tx, wx[xsel(11 + d)]), 11 + d);
+                        }
 #pragma unroll
                         for (int d = 2; d < 6; ++d)  // 15-18: y filter
                             nv[2 + d] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]), qy(d));
@@ -690,7 +708,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                     auto win = [&](int d, int m) -> float { return h[d][m + (S == 0 ? 1 : 0)]; };  // S = P
 #pragma unroll
                     for (int d = 0; d < 8; ++d) {
-                        const int q = gq(G, d);
+                        const int q = gq(G, d, SW);
                         const float* wz = XI + xifam(q) * 7;
                         float sum = win(d, 0) * wz[0];
 #pragma unroll

@@ -421,7 +421,8 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                 for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m) {
                     const int n = w + NRW * m;
                     if (n < NW16)
-                        lds_dma16(rfo, L + 256 * n, g16[m], lv);
+                        if (64 * n + lane < LY::USED / LY::GV)  // the last load's padding lanes stay idle
+                            lds_dma16(rfo, L + 256 * n, g16[m], lv);
                 }
                 return;
             }
@@ -517,9 +518,8 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             Sched sc{L0 - 1, k0};
             bool adv;
             int ho;
-            // xi weights of each step's target are staged into LDS one step ahead by the
-            // group B waves (one 64-lane load of the 56 weights of its row; the last 8 lanes
-            // read past the row and are not used)
+            // xi weights of each step's target are staged into LDS two steps ahead by the
+            // group B waves (one load of the 56 weights of its row by lanes 0-55)
             const auto rxi = rsrc_of(a.t.xiw, xi_bytes);
             unsigned p16 = 0;  // 16-byte staging of directions 0-2 (group B): this lane's granule
             if constexpr (G == 1 && W) {
@@ -532,17 +532,21 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             }
             auto fetch_xi = [&](int tk, int par) {
                 if constexpr (G == 1)  // an FP32 table whatever the population storage
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                        rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + ty) * 64), 4,
-                        (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
+                    if (lane < NXI * 7)
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + ty) * 64), 4,
+                            (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
             };
             // rows of directions 0, 1, 2 of target tk, staged by the group B wave of each row
             // for the group A hand-over (no z dependence)
             auto fetch012 = [&](int tk, int par) {
                 if constexpr (G == 1 && W) {
-                    // one 16-byte load: granules 0-15 direction 0, 16-33 direction 1 and 34-51
-                    // direction 2 (x halo from i0 - 4); the 12 padding lanes repeat granule 0
-                    lds_dma16(rfo, lds + OFF_P012 + (par * TY + ty) * P_ROW, p16, SZ * (unsigned)tk * (unsigned)plane);
+                    // one 16-byte load: direction 0 at the nodes, then 1 and 2 with the x halo
+                    // (FP32: granules 0-15, 16-33, 34-51; 16-bit: 0-7, 8-17, 18-27); the
+                    // remaining lanes stay idle
+                    constexpr int n012 = TX / LY::GV + 2 * (HXW / LY::GV);  // granules of the three rows
+                    if (lane < n012)
+                        lds_dma16(rfo, lds + OFF_P012 + (par * TY + ty) * P_ROW, p16, SZ * (unsigned)tk * (unsigned)plane);
                 } else if constexpr (G == 1) {
                     const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
                     float* P = lds + OFF_P012 + (par * TY + ty) * P_ROW;

@@ -123,6 +123,32 @@ def test_upload_download_roundtrip_and_state_resume(pkg, po):
     assert np.max(np.abs(s.f() - o.get_f())) < 1e-13
 
 
+# grids whose population arrays reach 4 MiB in the storage format, which the solver then
+# places an odd number of MiB apart (host copies move them row by row)
+@pytest.mark.parametrize("prec,grid,tol", [("fp64", (64, 128, 64), 1e-12), ("fp32", (128, 128, 64), 2e-5),
+                                           ("bf16", (128, 256, 64), 2e-2)])
+def test_upload_download_at_padded_stride(pkg, po, prec, grid, tol):
+    nx, ny, nz = grid
+    o, s = pair(pkg, po, nx=nx, ny=ny, nz=nz, precision=prec)
+    o.init_equilibrium(u0=(0.0, 0.03, 0.0), amp=1e-3, seed=17)
+    o.step(2)
+    fo = o.get_f()
+    s.upload_f(fo)
+    f1 = s.f()
+    if prec == "fp64":
+        assert np.array_equal(f1, fo)
+    else:
+        # storage rounding only, and a second round trip changes nothing
+        w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)[:, None, None, None]
+        rel = 2.0 ** (-23 if prec == "fp32" else -8)
+        assert np.all(np.abs(f1 - fo) <= rel * np.abs(fo - w) + 1e-15)
+        s.upload_f(f1)
+        assert np.array_equal(s.f(), f1)
+    o.step(3)
+    s.step(3)
+    compare(o, s, tol, f"{prec} {grid}")
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
 def test_reduced_precision_against_fp64_oracle(pkg, po, prec, tol):
     o, s = pair(pkg, po, precision=prec)

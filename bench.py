@@ -121,7 +121,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
-    ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled", "twopass"])
     ap.add_argument("--physics", default="stable-les", choices=sorted(PHYSICS))
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -137,7 +137,8 @@ def main() -> int:
         return 2
 
     pkg = importlib.import_module(PKG)
-    kern = {"auto": pkg.KERNEL_AUTO, "pernode": pkg.KERNEL_PERNODE, "tiled": pkg.KERNEL_TILED}[args.kernel]
+    kern = {"auto": pkg.KERNEL_AUTO, "pernode": pkg.KERNEL_PERNODE, "tiled": pkg.KERNEL_TILED,
+            "twopass": pkg.KERNEL_TWOPASS}[args.kernel]
     phys = PHYSICS[args.physics]
     p = pkg.default_params(**GRID, **phys, precision=args.precision, kernel=kern)
     s = pkg.Solver(p)
@@ -159,7 +160,7 @@ def main() -> int:
     avg_collide_s = collide_ms / 1e3 / args.steps
     bpl = BYTES_PER_LUP[args.precision]
     achieved = bpl * n / avg_collide_s / 1e9
-    ktag = "tiled" if s.kernel_in_use() == pkg.KERNEL_TILED else "pernode"
+    ktag = {pkg.KERNEL_TILED: "tiled", pkg.KERNEL_TWOPASS: "twopass"}.get(s.kernel_in_use(), "pernode")
     traffic = traffic_from_profiles(ktag, args.precision)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
@@ -172,7 +173,7 @@ def main() -> int:
                    "state_finite": finite, "event_ms_per_step": round(total_ms / args.steps, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_collide_{ktag}", "bytes_per_lup": bpl,
+                     "kernel": "k_hfilter+k_xi_collide" if ktag == "twopass" else f"k_collide_{ktag}", "bytes_per_lup": bpl,
                      "avg_launch_us": round(avg_collide_s * 1e6, 2)},
     }
     s.close()

@@ -46,9 +46,10 @@ extern "C" {
 #define ISLBM_MESH_STRETCH 1 /* a = tanh(stretch / 2)                                   */
 
 /* step kernel */
-#define ISLBM_KERNEL_AUTO 0      /* tiled kernel where supported, else per-node kernel   */
+#define ISLBM_KERNEL_AUTO 0      /* FP64: two-pass; else tiled where supported, else per-node */
 #define ISLBM_KERNEL_PERNODE 1   /* one thread per node, direct neighbour reads          */
 #define ISLBM_KERNEL_TILED 2     /* column-marching, LDS-staged, sum-factorised          */
+#define ISLBM_KERNEL_TWOPASS 3   /* FP64: x/y filter pass, then xi + walls + collision   */
 
 /* Run-time form of the reference's compile-time parameter set (variables.h:5-102). */
 typedef struct islbm_params {
@@ -177,7 +178,7 @@ const char* islbm_io_last_error(void);
 int islbm_scalars(islbm_solver* s, double out[8]);
 /* Steps taken since creation / the last init. */
 int islbm_step_count(islbm_solver* s, int64_t* steps);
-/* Kernel in use (ISLBM_KERNEL_PERNODE or ISLBM_KERNEL_TILED). */
+/* Kernel in use (ISLBM_KERNEL_PERNODE, ISLBM_KERNEL_TILED or ISLBM_KERNEL_TWOPASS). */
 int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel);
 
 /* Geometry table in unique-node form, as FP64 (names and layouts as the oracle's:

@@ -101,11 +101,18 @@ long long islbm::pop_stride(long long n, int bytes)
 
 namespace {
 
+// the two-pass FP64 kernel: AUTO's choice for FP64
+bool uses_twopass(const islbm_params& p)
+{
+    return p.precision == ISLBM_FP64 && (p.kernel == ISLBM_KERNEL_AUTO || p.kernel == ISLBM_KERNEL_TWOPASS);
+}
+
 size_t state_bytes(const islbm_params& p)
 {
     const size_t n = (size_t)p.nx * p.ny * p.nz;
     const size_t qs = (size_t)pop_stride((long long)n, (int)store_bytes(p.precision));
-    return 2 * NQ * qs * store_bytes(p.precision) + 4 * n * compute_bytes(p.precision) + table_bytes(p);
+    return 2 * NQ * qs * store_bytes(p.precision) + 4 * n * compute_bytes(p.precision) + table_bytes(p) +
+           (uses_twopass(p) ? 16 * n * sizeof(double) : 0);
 }
 
 __global__ void k_cast_f64(const double* in, float* out, long long n)
@@ -128,6 +135,7 @@ struct islbm_solver {
     void* f[2] = {nullptr, nullptr};
     int cur = 0;
     void* mac[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* hbuf = nullptr;  // two-pass FP64: filtered directions 3-18, [16][n]
     // tables (compute precision)
     void *xw0 = nullptr, *xw2 = nullptr, *yw0 = nullptr, *yw2 = nullptr, *xiw = nullptr;
     void *bq = nullptr, *bw = nullptr, *les = nullptr;
@@ -155,7 +163,7 @@ namespace {
 
 void free_all(islbm_solver* s)
 {
-    void* ptrs[] = {s->f[0], s->f[1], s->mac[0], s->xw0, s->xw2, s->yw0,
+    void* ptrs[] = {s->f[0], s->f[1], s->mac[0], s->hbuf, s->xw0, s->xw2, s->yw0,
                     s->yw2, s->xiw, s->bq, s->bw, s->les, s->bflag, s->fluxw, s->scal};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -319,6 +327,7 @@ StepLaunch make_launch(islbm_solver* s)
     L.u = s->mac[1];
     L.v = s->mac[2];
     L.w = s->mac[3];
+    L.hbuf = s->hbuf;
     L.tables = s->fp64 ? (const void*)&s->td : (const void*)&s->tf;
     L.force = s->scal + 0;
     L.rho_modify = s->scal + 1;
@@ -397,7 +406,7 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
     if (!pin || !out) return fail(ISLBM_ERR_ARG, "null argument");
     *out = nullptr;
     if (pin->precision < ISLBM_FP64 || pin->precision > ISLBM_FP16) return fail(ISLBM_ERR_ARG, "unknown precision");
-    if (pin->kernel < ISLBM_KERNEL_AUTO || pin->kernel > ISLBM_KERNEL_TILED) return fail(ISLBM_ERR_ARG, "unknown kernel");
+    if (pin->kernel < ISLBM_KERNEL_AUTO || pin->kernel > ISLBM_KERNEL_TWOPASS) return fail(ISLBM_ERR_ARG, "unknown kernel");
     islbm_solver* s = new islbm_solver();
     s->p = *pin;
     std::string msg = build_host_geometry(s->p, s->g);
@@ -417,12 +426,17 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
     for (int q = 0; q < NQ; ++q) s->scale[q] = dev_scale(s->p.precision, q);
     const LaunchGeom lg{s->p.nx, s->p.ny, s->p.nz, s->n, s->qs};
     if (s->p.kernel == ISLBM_KERNEL_AUTO)
-        s->kernel = tiled_supported(lg, s->store, s->fp64) ? ISLBM_KERNEL_TILED : ISLBM_KERNEL_PERNODE;
+        s->kernel = s->fp64 ? ISLBM_KERNEL_TWOPASS
+                            : (tiled_supported(lg, s->store, s->fp64) ? ISLBM_KERNEL_TILED : ISLBM_KERNEL_PERNODE);
     else
         s->kernel = s->p.kernel;
     if (s->kernel == ISLBM_KERNEL_TILED && !tiled_supported(lg, s->store, s->fp64)) {
         delete s;
         return fail(ISLBM_ERR_UNSUPPORTED, "tiled kernel does not support this grid/precision");
+    }
+    if (s->kernel == ISLBM_KERNEL_TWOPASS && !s->fp64) {
+        delete s;
+        return fail(ISLBM_ERR_UNSUPPORTED, "the two-pass kernel computes in FP64 only");
     }
     auto bail = [&](int rc) {
         free_all(s);
@@ -437,6 +451,10 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
     for (int b = 0; b < 2; ++b) {
         if (hipMalloc(&s->f[b], fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc f (%zu bytes)", fb));
         if (hipMemset(s->f[b], 0, fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset f"));
+    }
+    if (s->kernel == ISLBM_KERNEL_TWOPASS) {
+        const size_t hb = (size_t)16 * s->n * sizeof(double);
+        if (hipMalloc(&s->hbuf, hb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc filtered (%zu bytes)", hb));
     }
     // rho, u, v, w in one allocation, [4][nz][ny][nx]
     if (hipMalloc(&s->mac[0], 4 * mb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc macros"));
@@ -993,7 +1011,7 @@ int islbm_ref_init(const islbm_params* p)
         return fail(ISLBM_ERR_UNSUPPORTED, "the reference-named entry points take FP64 populations");
     ref_release();
     islbm_params q = *p;
-    q.kernel = ISLBM_KERNEL_PERNODE;
+    q.kernel = ISLBM_KERNEL_AUTO;  // FP64: the two-pass kernel
     int rc = islbm_create(&q, 0, &g_ref);
     if (rc) return rc;
     g_ref->auto_force = false;  // the reference's main calls Launch_ModifyForcingTerm itself

@@ -85,6 +85,7 @@ struct StepLaunch {
     const double* rho_modify;
     double tau, dt, cs_smag;
     const float* scale;   // per-direction 16-bit scale (device), or null
+    void* hbuf;           // two-pass FP64: horizontally filtered directions 3-18, [16][n]
 };
 hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L);
 hipError_t launch_stream_copy(hipStream_t st, const void* src, void* dst, long long bytes);

@@ -197,6 +197,198 @@ hipError_t launch_force_update(hipStream_t st, double* force, double* acc, doubl
 // Dispatch
 // ------------------------------------------------------------------------------------
 
+// ------------------------------------------------------------------------------------
+// Two-pass FP64 stream-collide.  The interpolated streaming is separable: every direction
+// is a 7-point filter in x and/or y of each level, then a 7-point xi sum over levels with
+// the target node's weights.  Pass 1 writes the horizontally filtered values H[q-3] of the
+// 16 directions 3-18 at every node; pass 2 takes each target's xi sums from H (7 levels of
+// one column), adds directions 0-2, applies the walls and the collision.  The sums keep
+// the reference's inner-to-outer order (x, y, then xi) except for 3 and 4, which the
+// reference sums xi-first (F3/F4_Intrpl7, interpolationHillISLBM.h:22-58); the difference
+// is FP64 rounding.
+// ------------------------------------------------------------------------------------
+
+// Node of this thread in the two-pass kernels.  A workgroup takes 256 consecutive nodes of
+// one level (a chunk of the (i, j) plane).  Workgroups are dealt to the 8 XCDs in turn
+// (workgroup b to XCD b % 8), each XCD with its own L2: the workgroups one XCD runs are given
+// a contiguous run of the (chunk, level) order, so that the neighbours that share a stencil
+// window run on the same L2 — rows of one plane for the x/y filter (chunks fastest), levels
+// of one column chunk for the xi sums (levels fastest).
+__host__ __device__ inline long long twopass_blocks(int nx, int ny, int nz)
+{
+    const long long nch = ((long long)nx * ny + 255) / 256;
+    return 8 * ((nch * nz + 7) / 8);
+}
+
+template <bool kLevelsFastest>
+__device__ inline bool twopass_node(int nx, int ny, int nz, int& i, int& j, int& k, long long& gid)
+{
+    const long long plane = (long long)nx * ny;
+    const long long nch = (plane + 255) / 256, nb = nch * nz;
+    const long long per = (nb + 7) / 8;
+    const long long b = blockIdx.x;
+    const long long L = (b % 8) * per + b / 8;
+    if (L >= nb) return false;
+    const long long c = kLevelsFastest ? L / nz : L % nch;
+    k = (int)(kLevelsFastest ? L % nz : L / nch);
+    const long long pn = c * 256 + threadIdx.x;
+    if (pn >= plane) return false;
+    i = (int)(pn % nx);
+    j = (int)(pn / nx);
+    gid = (long long)k * plane + pn;
+    return true;
+}
+
+// occupancy targets keep enough waves in flight for the gathers (the compiler would
+// otherwise hold every load of the unrolled sums in registers)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_hfilter(StepArgs<Store::F64> a,
+                                                                                         double* __restrict__ H)
+{
+    const int nx = a.nx, ny = a.ny;
+    int i, j, k;
+    long long gid;
+    if (!twopass_node<false>(nx, ny, a.nz, i, j, k, gid)) return;
+    const long long plane = (long long)nx * ny;
+    int xo[7], yo[7];
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+        xo[l] = wrap_index(i - 3 + l, nx);
+        yo[l] = wrap_index(j - 3 + l, ny) * nx;
+    }
+    const long long lev = (long long)k * plane;
+    // the x and y weights of the uniform mesh are those of every column and row (to FP64
+    // round-off): read once, as wave-uniform values
+    using cdouble = const __attribute__((address_space(4))) double;
+    const cdouble* X0 = (cdouble*)a.t.xw0;
+    const cdouble* X2 = (cdouble*)a.t.xw2;
+    const cdouble* Y0 = (cdouble*)a.t.yw0;
+    const cdouble* Y2 = (cdouble*)a.t.yw2;
+    const long long N = a.n;
+    // one direction at a time: the barrier after each store keeps the next direction's
+    // loads from being hoisted, so one 49-tap sum's registers are reused
+#pragma unroll
+    for (int q = 3; q < NQ; ++q) {
+        const double* p = a.fo + (long long)q * a.qs + lev;
+        const cdouble* wx = cx(q) > 0 ? X0 : X2;
+        const cdouble* wy = cy(q) > 0 ? Y0 : Y2;
+        double r;
+        if (cx(q) == 0 && cy(q) == 0) {  // 5, 6: at the node
+            r = p[(long long)j * nx + i];
+        } else if (cy(q) == 0) {  // 11-14: x only
+            const double* pr = p + (long long)j * nx;
+            r = pr[xo[0]] * wx[0];
+#pragma unroll
+            for (int l = 1; l < 7; ++l) r = r + pr[xo[l]] * wx[l];
+        } else if (cx(q) == 0) {  // 3, 4, 15-18: y only
+            const double* pc = p + i;
+            r = pc[yo[0]] * wy[0];
+#pragma unroll
+            for (int n = 1; n < 7; ++n) r = r + pc[yo[n]] * wy[n];
+        } else {  // 7-10: x inner, y outer
+            r = 0.0;
+#pragma unroll
+            for (int n = 0; n < 7; ++n) {
+                const double* pr = p + yo[n];
+                double in = pr[xo[0]] * wx[0];
+#pragma unroll
+                for (int l = 1; l < 7; ++l) in = in + pr[xo[l]] * wx[l];
+                r = (n == 0) ? in * wy[0] : r + in * wy[n];
+                asm volatile("" ::: "memory");  // one row of taps in flight at a time
+            }
+        }
+        H[(long long)(q - 3) * N + gid] = r;
+        asm volatile("" ::: "memory");
+    }
+}
+
+__global__ __launch_bounds__(256) void k_xi_collide(StepArgs<Store::F64> a, const double* __restrict__ H)
+{
+    using CD = Codec<Store::F64>;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    int i, j, k;
+    long long gid;
+    if (!twopass_node<true>(nx, ny, nz, i, j, k, gid)) return;
+    const long long plane = (long long)nx * ny;
+    const long long N = a.n;
+    int xo[7], yo[7];
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+        xo[l] = wrap_index(i - 3 + l, nx);
+        yo[l] = wrap_index(j - 3 + l, ny) * nx;
+    }
+    const long long row = (long long)k * plane + (long long)j * nx;
+    const long long node = row + i;
+    const long long col = (long long)xi_stencil_start(k, nz) * plane + (long long)j * nx + i;  // stencil base
+    const double* XI = a.t.xiw + ((long long)k * ny + j) * (NXI * 7);
+    auto pop = [&](int q) { return PopReader<Store::F64>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
+    // xi sum of direction q (H row q-3) with the weights of family fam
+    auto xi = [&](int q, int fam) {
+        const double* h = H + (long long)(q - 3) * N + col;
+        const double* wz = XI + fam * 7;
+        double r = h[0] * wz[0];
+#pragma unroll
+        for (int m = 1; m < 7; ++m) r = r + h[m * plane] * wz[m];
+        return r;
+    };
+    using cdouble = const __attribute__((address_space(4))) double;
+    auto ipx = [&](int q, const cdouble* wx) {
+        const double* p = a.fo + (long long)q * a.qs + row;
+        double r = p[xo[0]] * wx[0];
+#pragma unroll
+        for (int l = 1; l < 7; ++l) r = r + p[xo[l]] * wx[l];
+        return r;
+    };
+    // groups of four directions, so that the loads in flight stay within the registers
+    double F[NQ];
+    F[0] = pop(0)(node);
+    F[1] = ipx(1, (cdouble*)a.t.xw0);
+    F[2] = ipx(2, (cdouble*)a.t.xw2);
+    asm volatile("" ::: "memory");
+    F[3] = xi(3, XI3);
+    F[4] = xi(4, XI4);
+    F[5] = xi(5, XI5);
+    F[6] = xi(6, XI6);
+    asm volatile("" ::: "memory");
+    F[7] = xi(7, XI3);
+    F[8] = xi(8, XI3);
+    F[9] = xi(9, XI4);
+    F[10] = xi(10, XI4);
+    asm volatile("" ::: "memory");
+    F[11] = xi(11, XI5);
+    F[12] = xi(12, XI5);
+    F[13] = xi(13, XI6);
+    F[14] = xi(14, XI6);
+    asm volatile("" ::: "memory");
+    F[15] = xi(15, XI15);
+    F[16] = xi(16, XI16);
+    F[17] = xi(17, XI17);
+    F[18] = xi(18, XI18);
+    asm volatile("" ::: "memory");
+
+    apply_walls<Store::F64>(F, a, node, i, j, k, xo, yo, plane);
+
+    double tau_eff = a.tau;
+    double rho, ux, uy, uz;
+    collide_pairs<CD::kDev>(F, *a.rho_modify, a.dt, *a.force, a.les ? a.t.les_delta[(long long)k * ny + j] : 0.0, a.les,
+                            a.tau, a.cs, rho, ux, uy, uz, tau_eff);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) CD::st(a.fn + (long long)q * a.qs, node, F[q], a.scale[q]);
+    a.rho[node] = rho;
+    a.u[node] = ux;
+    a.v[node] = uy;
+    a.w[node] = uz;
+}
+
+static hipError_t launch_twopass(hipStream_t st, const StepLaunch& L)
+{
+    if (L.store != Store::F64 || !L.hbuf) return hipErrorInvalidValue;
+    StepArgs<Store::F64> a = make_step_args<Store::F64>(L);
+    const unsigned nb = (unsigned)twopass_blocks(L.g.nx, L.g.ny, L.g.nz);
+    hipLaunchKernelGGL(k_hfilter, dim3(nb), dim3(256), 0, st, a, (double*)L.hbuf);
+    hipLaunchKernelGGL(k_xi_collide, dim3(nb), dim3(256), 0, st, a, (const double*)L.hbuf);
+    return hipGetLastError();
+}
+
 template <Store ST>
 static hipError_t launch_pernode(hipStream_t st, const StepLaunch& L)
 {
@@ -211,6 +403,7 @@ hipError_t launch_tiled(hipStream_t st, const StepLaunch& L);  // islbm_tiled.hi
 hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L)
 {
     if (L.kernel == ISLBM_KERNEL_TILED) return launch_tiled(st, L);
+    if (L.kernel == ISLBM_KERNEL_TWOPASS) return launch_twopass(st, L);
     switch (L.store) {
     case Store::F64: return launch_pernode<Store::F64>(st, L);
     case Store::F32: return launch_pernode<Store::F32>(st, L);

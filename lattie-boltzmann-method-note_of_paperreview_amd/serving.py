@@ -84,7 +84,7 @@ def run_batched(reqs, rate: float, seed: int = 7, precision: str = "fp32", max_a
             "wall_s": float(t_end), "aggregate_mlups": float(lups / t_end / 1e6) if t_end > 0 else None}
 
 
-def run_sequential(reqs, precision: str = "fp64", kernel: int = islbm.KERNEL_PERNODE):
+def run_sequential(reqs, precision: str = "fp64", kernel: int = islbm.KERNEL_AUTO):
     """One solve at a time on the GPU (create, init, step, download), for the comparison."""
     t0 = time.perf_counter()
     lups = 0.0

@@ -49,14 +49,17 @@ def test_geometry_tables_match_oracle(pkg, po, mode):
 # grids, e.g. ny = 24, diverge in the oracle as well)
 @pytest.mark.parametrize("mode", [dict(), dict(mesh_mode=1, stretch=1.2), dict(flat=1), dict(ny=40, nx=20, nz=24),
                                   dict(hill_alpha=0.8, ly=5.142 + 3.858 * 0.8, ny=48)])
-def test_fp64_step_parity_small(pkg, po, mode):
-    o, s = pair(pkg, po, **mode)
+@pytest.mark.parametrize("kernel", ["pernode", "twopass"])
+def test_fp64_step_parity_small(pkg, po, mode, kernel):
+    kern = {"pernode": pkg.KERNEL_PERNODE, "twopass": pkg.KERNEL_TWOPASS}[kernel]
+    o, s = pair(pkg, po, kernel=kern, **mode)
+    assert s.kernel_in_use() == kern
     for x in (o, s):
         x.init_equilibrium(u0=(0.0, 0.0583, 0.0), amp=1e-3, seed=7)
     for n in (1, 9, 40):
         o.step(n)
         s.step(n)
-        compare(o, s, TOL64, f"{mode} +{n} steps")
+        compare(o, s, TOL64, f"{kernel} {mode} +{n} steps")
 
 
 def test_fp64_parity_reference_form_32x64x64_1000_steps(pkg, po):
@@ -246,6 +249,28 @@ def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
     o.step(60)
     s.step(60)
     compare(o, s, tol, f"tiled {prec} {shape}")
+
+
+def test_fp64_auto_is_twopass_and_equals_pernode(pkg):
+    """FP64 solvers take the two-pass kernel by default; it sums the same taps as the
+    per-node kernel in another order (3 and 4 filtered before the xi sum), so the two agree
+    to FP64 round-off after many steps, at a shape with partial workgroups."""
+    kw = dict(nx=40, ny=48, nz=36, tau=0.6833, re=50.0, mesh_mode=0, u_ref=0.0, precision="fp64", cs_smag=0.1)
+    a = pkg.Solver(pkg.default_params(**kw))
+    b = pkg.Solver(pkg.default_params(kernel=pkg.KERNEL_PERNODE, **kw))
+    assert a.kernel_in_use() == pkg.KERNEL_TWOPASS
+    for x in (a, b):
+        x.init_equilibrium(u0=(0.0, 0.0583, 0.0), amp=1e-3, seed=11)
+        x.step(30)
+    fa, fb = a.f(), b.f()
+    d = float(np.max(np.abs(fa - fb)))
+    print(f"twopass vs pernode FP64, 30 steps: max |df| {d:.2e}", flush=True)
+    assert d < 1e-13
+
+
+def test_twopass_is_refused_for_reduced_precision(pkg):
+    with pytest.raises(pkg.ISLBMError):
+        pkg.Solver(pkg.default_params(nx=16, ny=32, nz=32, precision="fp32", kernel=pkg.KERNEL_TWOPASS))
 
 
 def test_tiled_equals_pernode_fp32(pkg, po):

@@ -112,7 +112,7 @@ size_t state_bytes(const islbm_params& p)
     const size_t n = (size_t)p.nx * p.ny * p.nz;
     const size_t qs = (size_t)pop_stride((long long)n, (int)store_bytes(p.precision));
     return 2 * NQ * qs * store_bytes(p.precision) + 4 * n * compute_bytes(p.precision) + table_bytes(p) +
-           (uses_twopass(p) ? 16 * n * sizeof(double) : 0);
+           (uses_twopass(p) ? 16 * qs * sizeof(double) : 0);  // H rows at the population stride
 }
 
 __global__ void k_cast_f64(const double* in, float* out, long long n)
@@ -453,7 +453,7 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
         if (hipMemset(s->f[b], 0, fb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMemset f"));
     }
     if (s->kernel == ISLBM_KERNEL_TWOPASS) {
-        const size_t hb = (size_t)16 * s->n * sizeof(double);
+        const size_t hb = (size_t)16 * s->qs * sizeof(double);
         if (hipMalloc(&s->hbuf, hb) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "hipMalloc filtered (%zu bytes)", hb));
     }
     // rho, u, v, w in one allocation, [4][nz][ny][nx]

@@ -201,11 +201,12 @@ hipError_t launch_force_update(hipStream_t st, double* force, double* acc, doubl
 // Two-pass FP64 stream-collide.  The interpolated streaming is separable: every direction
 // is a 7-point filter in x and/or y of each level, then a 7-point xi sum over levels with
 // the target node's weights.  Pass 1 writes the horizontally filtered values H[q-3] of the
-// 16 directions 3-18 at every node; pass 2 takes each target's xi sums from H (7 levels of
-// one column), adds directions 0-2, applies the walls and the collision.  The sums keep
-// the reference's inner-to-outer order (x, y, then xi) except for 3 and 4, which the
-// reference sums xi-first (F3/F4_Intrpl7, interpolationHillISLBM.h:22-58); the difference
-// is FP64 rounding.
+// 16 directions 3-18 at every node (rows at the population stride); pass 2 takes each
+// target's xi sums from H (7 levels of one column) and directions 1, 2 into the new buffer;
+// the wall levels and then the collision update it in place.  The sums keep the
+// reference's inner-to-outer order (x, y, then xi) except for 3 and 4, which the reference
+// sums xi-first (F3/F4_Intrpl7, interpolationHillISLBM.h:22-58); the difference is FP64
+// rounding.
 // ------------------------------------------------------------------------------------
 
 // Node of this thread in the two-pass kernels.  A workgroup takes 256 consecutive nodes of
@@ -214,24 +215,24 @@ hipError_t launch_force_update(hipStream_t st, double* force, double* acc, doubl
 // a contiguous run of the (chunk, level) order, so that the neighbours that share a stencil
 // window run on the same L2 — rows of one plane for the x/y filter (chunks fastest), levels
 // of one column chunk for the xi sums (levels fastest).
-__host__ __device__ inline long long twopass_blocks(int nx, int ny, int nz)
+__host__ __device__ inline long long twopass_blocks(int nx, int ny, int nz, int npt = 1)
 {
-    const long long nch = ((long long)nx * ny + 255) / 256;
+    const long long nch = ((long long)nx * ny + 256 * npt - 1) / (256 * npt);
     return 8 * ((nch * nz + 7) / 8);
 }
 
-template <bool kLevelsFastest>
+template <bool kLevelsFastest, int NPT = 1>
 __device__ inline bool twopass_node(int nx, int ny, int nz, int& i, int& j, int& k, long long& gid)
 {
     const long long plane = (long long)nx * ny;
-    const long long nch = (plane + 255) / 256, nb = nch * nz;
+    const long long nch = (plane + 256 * NPT - 1) / (256 * NPT), nb = nch * nz;
     const long long per = (nb + 7) / 8;
     const long long b = blockIdx.x;
     const long long L = (b % 8) * per + b / 8;
     if (L >= nb) return false;
     const long long c = kLevelsFastest ? L / nz : L % nch;
     k = (int)(kLevelsFastest ? L % nz : L / nch);
-    const long long pn = c * 256 + threadIdx.x;
+    const long long pn = (c * 256 + threadIdx.x) * NPT;
     if (pn >= plane) return false;
     i = (int)(pn % nx);
     j = (int)(pn / nx);
@@ -263,7 +264,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const cdouble* X2 = (cdouble*)a.t.xw2;
     const cdouble* Y0 = (cdouble*)a.t.yw0;
     const cdouble* Y2 = (cdouble*)a.t.yw2;
-    const long long N = a.n;
     // one direction at a time: the barrier after each store keeps the next direction's
     // loads from being hoisted, so one 49-tap sum's registers are reused
 #pragma unroll
@@ -296,20 +296,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 asm volatile("" ::: "memory");  // one row of taps in flight at a time
             }
         }
-        H[(long long)(q - 3) * N + gid] = r;
+        H[(long long)(q - 3) * a.qs + gid] = r;
         asm volatile("" ::: "memory");
     }
 }
 
-__global__ __launch_bounds__(256) void k_xi_collide(StepArgs<Store::F64> a, const double* __restrict__ H)
+// Pass 2: the xi sums of directions 3-18 from H (7 levels of one column) and directions 1, 2;
+// the streamed populations go to the new buffer, which the wall and collision passes then
+// update in place.  Light in registers, so that many waves keep the gathers in flight.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xi_stream(StepArgs<Store::F64> a,
+                                                                                           const double* __restrict__ H)
 {
-    using CD = Codec<Store::F64>;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     int i, j, k;
     long long gid;
     if (!twopass_node<true>(nx, ny, nz, i, j, k, gid)) return;
     const long long plane = (long long)nx * ny;
-    const long long N = a.n;
     int xo[7], yo[7];
 #pragma unroll
     for (int l = 0; l < 7; ++l) {
@@ -320,10 +322,9 @@ __global__ __launch_bounds__(256) void k_xi_collide(StepArgs<Store::F64> a, cons
     const long long node = row + i;
     const long long col = (long long)xi_stencil_start(k, nz) * plane + (long long)j * nx + i;  // stencil base
     const double* XI = a.t.xiw + ((long long)k * ny + j) * (NXI * 7);
-    auto pop = [&](int q) { return PopReader<Store::F64>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
     // xi sum of direction q (H row q-3) with the weights of family fam
     auto xi = [&](int q, int fam) {
-        const double* h = H + (long long)(q - 3) * N + col;
+        const double* h = H + (long long)(q - 3) * a.qs + col;
         const double* wz = XI + fam * 7;
         double r = h[0] * wz[0];
 #pragma unroll
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(256) void k_xi_collide(StepArgs<Store::F64> a, cons
     };
     // groups of four directions, so that the loads in flight stay within the registers
     double F[NQ];
-    F[0] = pop(0)(node);
+    F[0] = 0.0;  // the collision pass reads direction 0 at the node
     F[1] = ipx(1, (cdouble*)a.t.xw0);
     F[2] = ipx(2, (cdouble*)a.t.xw2);
     asm volatile("" ::: "memory");
@@ -364,12 +365,52 @@ __global__ __launch_bounds__(256) void k_xi_collide(StepArgs<Store::F64> a, cons
     F[17] = xi(17, XI17);
     F[18] = xi(18, XI18);
     asm volatile("" ::: "memory");
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) a.fn[(long long)q * a.qs + node] = F[q];
+}
 
+// The walls (bounce-back at k = 0 and nz - 1, BFL at k <= 1) on the streamed populations of
+// the three wall levels, in place; they read only the old populations.
+__global__ __launch_bounds__(256) void k_walls_inplace(StepArgs<Store::F64> a)
+{
+    const long long plane = (long long)a.nx * a.ny;
+    const long long pn = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pn >= plane) return;
+    const int k = blockIdx.y == 2 ? a.nz - 1 : (int)blockIdx.y;
+    const int i = (int)(pn % a.nx), j = (int)(pn / a.nx);
+    const long long node = (long long)k * plane + pn;
+    int xo[7], yo[7];
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+        xo[l] = wrap_index(i - 3 + l, a.nx);
+        yo[l] = wrap_index(j - 3 + l, a.ny) * a.nx;
+    }
+    double F[NQ];
+    F[0] = 0.0;
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) F[q] = a.fn[(long long)q * a.qs + node];
     apply_walls<Store::F64>(F, a, node, i, j, k, xo, yo, plane);
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) a.fn[(long long)q * a.qs + node] = F[q];
+}
 
+// Pass 3: the MRT(+LES) collision at each node, in place on the new buffer (direction 0
+// from the old one).
+__global__ __launch_bounds__(256) void k_collide_inplace(StepArgs<Store::F64> a)
+{
+    using CD = Codec<Store::F64>;
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= a.n) return;
+    const long long plane = (long long)a.nx * a.ny;
+    const int k = (int)(node / plane);
+    const int j = (int)((node - (long long)k * plane) / a.nx);
+    double F[NQ];
+    F[0] = a.fo[node];
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) F[q] = a.fn[(long long)q * a.qs + node];
     double tau_eff = a.tau;
     double rho, ux, uy, uz;
-    collide_pairs<CD::kDev>(F, *a.rho_modify, a.dt, *a.force, a.les ? a.t.les_delta[(long long)k * ny + j] : 0.0, a.les,
+    collide_pairs<CD::kDev>(F, *a.rho_modify, a.dt, *a.force, a.les ? a.t.les_delta[(long long)k * a.ny + j] : 0.0, a.les,
                             a.tau, a.cs, rho, ux, uy, uz, tau_eff);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) CD::st(a.fn + (long long)q * a.qs, node, F[q], a.scale[q]);
@@ -379,13 +420,189 @@ __global__ __launch_bounds__(256) void k_xi_collide(StepArgs<Store::F64> a, cons
     a.w[node] = uz;
 }
 
+// ------------------------------------------------------------------------------------
+// Paired forms (even nx): each thread takes two nodes (i, i+1) of one row, i even, and every
+// tap is an aligned 16-byte load of a pair.  The x window of the pair is the five pairs
+// i-4 .. i+5, which wrap as whole pairs (nx even); node i takes values 1-7 of it, node i+1
+// values 2-8.  Same sums, same order as the one-node kernels above.
+// ------------------------------------------------------------------------------------
+using d2 = HIP_vector_type<double, 2>;
+
+__device__ inline d2 ld2(const double* p) { return *reinterpret_cast<const d2*>(p); }
+__device__ inline void st2(double* p, double a, double b) { *reinterpret_cast<d2*>(p) = d2(a, b); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_hfilter2(StepArgs<Store::F64> a,
+                                                                                          double* __restrict__ H)
+{
+    const int nx = a.nx, ny = a.ny;
+    int i, j, k;
+    long long gid;
+    if (!twopass_node<false, 2>(nx, ny, a.nz, i, j, k, gid)) return;
+    const long long plane = (long long)nx * ny;
+    int xp[5], yo[7];  // pair offsets of the x window; row offsets of the y taps
+#pragma unroll
+    for (int m = 0; m < 5; ++m) xp[m] = 2 * wrap_index(i / 2 - 2 + m, nx / 2);
+#pragma unroll
+    for (int l = 0; l < 7; ++l) yo[l] = wrap_index(j - 3 + l, ny) * nx;
+    const long long lev = (long long)k * plane;
+    using cdouble = const __attribute__((address_space(4))) double;
+    const cdouble* X0 = (cdouble*)a.t.xw0;
+    const cdouble* X2 = (cdouble*)a.t.xw2;
+    const cdouble* Y0 = (cdouble*)a.t.yw0;
+    const cdouble* Y2 = (cdouble*)a.t.yw2;
+    // x filter of one row for both nodes
+    auto xrow = [&](const double* pr, const cdouble* wx, double& r0, double& r1) {
+        double w[10];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const d2 v = ld2(pr + xp[m]);
+            w[2 * m] = v.x;
+            w[2 * m + 1] = v.y;
+        }
+        r0 = w[1] * wx[0];
+        r1 = w[2] * wx[0];
+#pragma unroll
+        for (int l = 1; l < 7; ++l) {
+            r0 = r0 + w[1 + l] * wx[l];
+            r1 = r1 + w[2 + l] * wx[l];
+        }
+    };
+#pragma unroll
+    for (int q = 3; q < NQ; ++q) {
+        const double* p = a.fo + (long long)q * a.qs + lev;
+        const cdouble* wx = cx(q) > 0 ? X0 : X2;
+        const cdouble* wy = cy(q) > 0 ? Y0 : Y2;
+        double r0, r1;
+        if (cx(q) == 0 && cy(q) == 0) {  // 5, 6: at the nodes
+            const d2 v = ld2(p + (long long)j * nx + i);
+            r0 = v.x;
+            r1 = v.y;
+        } else if (cy(q) == 0) {  // 11-14: x only
+            xrow(p + (long long)j * nx, wx, r0, r1);
+        } else if (cx(q) == 0) {  // 3, 4, 15-18: y only
+            const double* pc = p + i;
+            d2 v = ld2(pc + yo[0]);
+            r0 = v.x * wy[0];
+            r1 = v.y * wy[0];
+#pragma unroll
+            for (int n = 1; n < 7; ++n) {
+                v = ld2(pc + yo[n]);
+                r0 = r0 + v.x * wy[n];
+                r1 = r1 + v.y * wy[n];
+            }
+        } else {  // 7-10: x inner, y outer
+            r0 = r1 = 0.0;
+#pragma unroll
+            for (int n = 0; n < 7; ++n) {
+                double i0, i1;
+                xrow(p + yo[n], wx, i0, i1);
+                r0 = (n == 0) ? i0 * wy[0] : r0 + i0 * wy[n];
+                r1 = (n == 0) ? i1 * wy[0] : r1 + i1 * wy[n];
+                asm volatile("" ::: "memory");  // one row of taps in flight at a time
+            }
+        }
+        st2(H + (long long)(q - 3) * a.qs + gid, r0, r1);
+        asm volatile("" ::: "memory");
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xi_stream2(StepArgs<Store::F64> a,
+                                                                                            const double* __restrict__ H)
+{
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    int i, j, k;
+    long long gid;
+    if (!twopass_node<true, 2>(nx, ny, nz, i, j, k, gid)) return;
+    const long long plane = (long long)nx * ny;
+    int xp[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) xp[m] = 2 * wrap_index(i / 2 - 2 + m, nx / 2);
+    const long long row = (long long)k * plane + (long long)j * nx;
+    const long long node = row + i;
+    const long long col = (long long)xi_stencil_start(k, nz) * plane + (long long)j * nx + i;  // stencil base
+    const double* XI = a.t.xiw + ((long long)k * ny + j) * (NXI * 7);
+    double F0[NQ], F1[NQ];
+    auto xi = [&](int q, int fam) {
+        const double* h = H + (long long)(q - 3) * a.qs + col;
+        const double* wz = XI + fam * 7;
+        d2 v = ld2(h);
+        double r0 = v.x * wz[0], r1 = v.y * wz[0];
+#pragma unroll
+        for (int m = 1; m < 7; ++m) {
+            v = ld2(h + m * plane);
+            r0 = r0 + v.x * wz[m];
+            r1 = r1 + v.y * wz[m];
+        }
+        F0[q] = r0;
+        F1[q] = r1;
+    };
+    using cdouble = const __attribute__((address_space(4))) double;
+    auto ipx = [&](int q, const cdouble* wx) {
+        const double* pr = a.fo + (long long)q * a.qs + row;
+        double w[10];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const d2 v = ld2(pr + xp[m]);
+            w[2 * m] = v.x;
+            w[2 * m + 1] = v.y;
+        }
+        double r0 = w[1] * wx[0], r1 = w[2] * wx[0];
+#pragma unroll
+        for (int l = 1; l < 7; ++l) {
+            r0 = r0 + w[1 + l] * wx[l];
+            r1 = r1 + w[2 + l] * wx[l];
+        }
+        F0[q] = r0;
+        F1[q] = r1;
+    };
+    // groups of two directions, so that the loads in flight stay within the registers
+    ipx(1, (cdouble*)a.t.xw0);
+    ipx(2, (cdouble*)a.t.xw2);
+    asm volatile("" ::: "memory");
+    xi(3, XI3);
+    xi(4, XI4);
+    asm volatile("" ::: "memory");
+    xi(5, XI5);
+    xi(6, XI6);
+    asm volatile("" ::: "memory");
+    xi(7, XI3);
+    xi(8, XI3);
+    asm volatile("" ::: "memory");
+    xi(9, XI4);
+    xi(10, XI4);
+    asm volatile("" ::: "memory");
+    xi(11, XI5);
+    xi(12, XI5);
+    asm volatile("" ::: "memory");
+    xi(13, XI6);
+    xi(14, XI6);
+    asm volatile("" ::: "memory");
+    xi(15, XI15);
+    xi(16, XI16);
+    asm volatile("" ::: "memory");
+    xi(17, XI17);
+    xi(18, XI18);
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) st2(a.fn + (long long)q * a.qs + node, F0[q], F1[q]);
+}
+
 static hipError_t launch_twopass(hipStream_t st, const StepLaunch& L)
 {
     if (L.store != Store::F64 || !L.hbuf) return hipErrorInvalidValue;
     StepArgs<Store::F64> a = make_step_args<Store::F64>(L);
-    const unsigned nb = (unsigned)twopass_blocks(L.g.nx, L.g.ny, L.g.nz);
-    hipLaunchKernelGGL(k_hfilter, dim3(nb), dim3(256), 0, st, a, (double*)L.hbuf);
-    hipLaunchKernelGGL(k_xi_collide, dim3(nb), dim3(256), 0, st, a, (const double*)L.hbuf);
+    if (L.g.nx % 2 == 0) {  // paired forms: 16-byte taps
+        const unsigned nb = (unsigned)twopass_blocks(L.g.nx, L.g.ny, L.g.nz, 2);
+        hipLaunchKernelGGL(k_hfilter2, dim3(nb), dim3(256), 0, st, a, (double*)L.hbuf);
+        hipLaunchKernelGGL(k_xi_stream2, dim3(nb), dim3(256), 0, st, a, (const double*)L.hbuf);
+    } else {
+        const unsigned nb = (unsigned)twopass_blocks(L.g.nx, L.g.ny, L.g.nz);
+        hipLaunchKernelGGL(k_hfilter, dim3(nb), dim3(256), 0, st, a, (double*)L.hbuf);
+        hipLaunchKernelGGL(k_xi_stream, dim3(nb), dim3(256), 0, st, a, (const double*)L.hbuf);
+    }
+    const long long plane = (long long)L.g.nx * L.g.ny;
+    hipLaunchKernelGGL(k_walls_inplace, dim3((unsigned)((plane + 255) / 256), 3), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_collide_inplace, dim3((unsigned)((L.g.n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

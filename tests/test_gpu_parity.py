@@ -47,8 +47,9 @@ def test_geometry_tables_match_oracle(pkg, po, mode):
 
 # shapes on which the reference scheme stays bounded for these 50 steps (coarser streamwise
 # grids, e.g. ny = 24, diverge in the oracle as well)
+# (nx = 21: the two-pass kernel's one-node forms; even nx takes its paired forms)
 @pytest.mark.parametrize("mode", [dict(), dict(mesh_mode=1, stretch=1.2), dict(flat=1), dict(ny=40, nx=20, nz=24),
-                                  dict(hill_alpha=0.8, ly=5.142 + 3.858 * 0.8, ny=48)])
+                                  dict(hill_alpha=0.8, ly=5.142 + 3.858 * 0.8, ny=48), dict(nx=21)])
 @pytest.mark.parametrize("kernel", ["pernode", "twopass"])
 def test_fp64_step_parity_small(pkg, po, mode, kernel):
     kern = {"pernode": pkg.KERNEL_PERNODE, "twopass": pkg.KERNEL_TWOPASS}[kernel]
@@ -251,11 +252,13 @@ def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
     compare(o, s, tol, f"tiled {prec} {shape}")
 
 
-def test_fp64_auto_is_twopass_and_equals_pernode(pkg):
+@pytest.mark.parametrize("nx", [40, 39])
+def test_fp64_auto_is_twopass_and_equals_pernode(pkg, nx):
     """FP64 solvers take the two-pass kernel by default; it sums the same taps as the
     per-node kernel in another order (3 and 4 filtered before the xi sum), so the two agree
-    to FP64 round-off after many steps, at a shape with partial workgroups."""
-    kw = dict(nx=40, ny=48, nz=36, tau=0.6833, re=50.0, mesh_mode=0, u_ref=0.0, precision="fp64", cs_smag=0.1)
+    to FP64 round-off after many steps, at shapes with partial workgroups (paired forms at
+    even nx, one-node forms at odd nx)."""
+    kw = dict(nx=nx, ny=48, nz=36, tau=0.6833, re=50.0, mesh_mode=0, u_ref=0.0, precision="fp64", cs_smag=0.1)
     a = pkg.Solver(pkg.default_params(**kw))
     b = pkg.Solver(pkg.default_params(kernel=pkg.KERNEL_PERNODE, **kw))
     assert a.kernel_in_use() == pkg.KERNEL_TWOPASS

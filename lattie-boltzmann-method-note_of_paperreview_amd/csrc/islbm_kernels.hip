@@ -203,7 +203,7 @@ hipError_t launch_force_update(hipStream_t st, double* force, double* acc, doubl
 // the target node's weights.  Pass 1 writes the horizontally filtered values H[q-3] of the
 // 16 directions 3-18 at every node (rows at the population stride); pass 2 takes each
 // target's xi sums from H (7 levels of one column) and directions 1, 2 into the new buffer;
-// the wall levels and then the collision update it in place.  The sums keep the
+// pass 3 applies the walls and the collision in place.  The sums keep the
 // reference's inner-to-outer order (x, y, then xi) except for 3 and 4, which the reference
 // sums xi-first (F3/F4_Intrpl7, interpolationHillISLBM.h:22-58); the difference is FP64
 // rounding.
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // Pass 2: the xi sums of directions 3-18 from H (7 levels of one column) and directions 1, 2;
-// the streamed populations go to the new buffer, which the wall and collision passes then
+// the streamed populations go to the new buffer, which the wall and collision pass then
 // update in place.  Light in registers, so that many waves keep the gathers in flight.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xi_stream(StepArgs<Store::F64> a,
                                                                                            const double* __restrict__ H)
@@ -369,33 +369,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int q = 1; q < NQ; ++q) a.fn[(long long)q * a.qs + node] = F[q];
 }
 
-// The walls (bounce-back at k = 0 and nz - 1, BFL at k <= 1) on the streamed populations of
-// the three wall levels, in place; they read only the old populations.
-__global__ __launch_bounds__(256) void k_walls_inplace(StepArgs<Store::F64> a)
-{
-    const long long plane = (long long)a.nx * a.ny;
-    const long long pn = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (pn >= plane) return;
-    const int k = blockIdx.y == 2 ? a.nz - 1 : (int)blockIdx.y;
-    const int i = (int)(pn % a.nx), j = (int)(pn / a.nx);
-    const long long node = (long long)k * plane + pn;
-    int xo[7], yo[7];
-#pragma unroll
-    for (int l = 0; l < 7; ++l) {
-        xo[l] = wrap_index(i - 3 + l, a.nx);
-        yo[l] = wrap_index(j - 3 + l, a.ny) * a.nx;
-    }
-    double F[NQ];
-    F[0] = 0.0;
-#pragma unroll
-    for (int q = 1; q < NQ; ++q) F[q] = a.fn[(long long)q * a.qs + node];
-    apply_walls<Store::F64>(F, a, node, i, j, k, xo, yo, plane);
-#pragma unroll
-    for (int q = 1; q < NQ; ++q) a.fn[(long long)q * a.qs + node] = F[q];
-}
-
-// Pass 3: the MRT(+LES) collision at each node, in place on the new buffer (direction 0
-// from the old one).
+// Pass 3: the walls (bounce-back at k = 0 and nz - 1, BFL at k <= 1; they read only the old
+// populations) and the MRT(+LES) collision at each node, in place on the new buffer
+// (direction 0 from the old one).
 __global__ __launch_bounds__(256) void k_collide_inplace(StepArgs<Store::F64> a)
 {
     using CD = Codec<Store::F64>;
@@ -408,6 +384,16 @@ __global__ __launch_bounds__(256) void k_collide_inplace(StepArgs<Store::F64> a)
     F[0] = a.fo[node];
 #pragma unroll
     for (int q = 1; q < NQ; ++q) F[q] = a.fn[(long long)q * a.qs + node];
+    if (k <= 1 || k == a.nz - 1) {
+        const int i = (int)(node - (long long)k * plane - (long long)j * a.nx);
+        int xo[7], yo[7];
+#pragma unroll
+        for (int l = 0; l < 7; ++l) {
+            xo[l] = wrap_index(i - 3 + l, a.nx);
+            yo[l] = wrap_index(j - 3 + l, a.ny) * a.nx;
+        }
+        apply_walls<Store::F64>(F, a, node, i, j, k, xo, yo, plane);
+    }
     double tau_eff = a.tau;
     double rho, ux, uy, uz;
     collide_pairs<CD::kDev>(F, *a.rho_modify, a.dt, *a.force, a.les ? a.t.les_delta[(long long)k * a.ny + j] : 0.0, a.les,
@@ -587,6 +573,111 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int q = 1; q < NQ; ++q) st2(a.fn + (long long)q * a.qs + node, F0[q], F1[q]);
 }
 
+// LDS form of the xi pass (even nx): a workgroup takes one chunk of 128 nodes (64 pairs) of
+// the (i, j) plane at 8 consecutive levels, one wave per level.  For each group of 4
+// directions the levels the 8 stencils share (at most 14) are staged once into LDS, so each
+// value of H is requested from L2 about 1.75 times instead of 7.
+constexpr int kXiLevels = 8;    // levels per workgroup (one wave each)
+constexpr int kXiStage = 14;    // staged levels: 8 + 6
+constexpr int kXiDirs = 4;      // directions per staging round
+
+__host__ __device__ inline long long xi_lds_blocks(int nx, int ny, int nz)
+{
+    const long long nch = ((long long)nx * ny + 127) / 128;
+    const long long ng = (nz + kXiLevels - 1) / kXiLevels;
+    return 8 * ((nch * ng + 7) / 8);
+}
+
+__global__ __launch_bounds__(64 * kXiLevels) void k_xi_stream_lds(StepArgs<Store::F64> a, const double* __restrict__ H)
+{
+    __shared__ d2 S[kXiDirs][kXiStage][64];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const long long plane = (long long)nx * ny;
+    const long long nch = (plane + 127) / 128;
+    const long long ng = (nz + kXiLevels - 1) / kXiLevels, nb = nch * ng;
+    const long long per = (nb + 7) / 8;
+    const long long b = blockIdx.x;
+    const long long L = (b % 8) * per + b / 8;  // XCD-contiguous run, levels fastest
+    if (L >= nb) return;                         // whole workgroup: no barrier skipped
+    const long long c = L / ng;
+    const int k0 = (int)(L % ng) * kXiLevels;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = k0 + wv;
+    const long long pn = c * 128 + 2 * lane;
+    const bool active = k < nz && pn < plane;
+    const int klast = min(k0 + kXiLevels - 1, nz - 1);
+    const int ks0 = xi_stencil_start(k0, nz);
+    const int nl = xi_stencil_start(klast, nz) + 7 - ks0;  // staged levels, <= kXiStage
+    const int kk = active ? k : klast;
+    const int i = (int)(pn % nx), j = (int)(pn / nx);
+    const int sl = xi_stencil_start(kk, nz) - ks0;         // first staged level of this stencil
+    const double* XI = a.t.xiw + ((long long)kk * ny + (active ? j : 0)) * (NXI * 7);
+    const long long row = (long long)kk * plane + (long long)j * nx;
+    const long long node = row + i;
+    double F0[NQ], F1[NQ];
+    if (active) {  // directions 1 and 2: x filters of the old populations at the node's row
+        int xp[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) xp[m] = 2 * wrap_index(i / 2 - 2 + m, nx / 2);
+        using cdouble = const __attribute__((address_space(4))) double;
+#pragma unroll
+        for (int q = 1; q <= 2; ++q) {
+            const cdouble* wx = (cdouble*)(q == 1 ? a.t.xw0 : a.t.xw2);
+            const double* pr = a.fo + (long long)q * a.qs + row;
+            double w[10];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const d2 v = ld2(pr + xp[m]);
+                w[2 * m] = v.x;
+                w[2 * m + 1] = v.y;
+            }
+            double r0 = w[1] * wx[0], r1 = w[2] * wx[0];
+#pragma unroll
+            for (int l = 1; l < 7; ++l) {
+                r0 = r0 + w[1 + l] * wx[l];
+                r1 = r1 + w[2 + l] * wx[l];
+            }
+            F0[q] = r0;
+            F1[q] = r1;
+        }
+    }
+    constexpr int fam[NQ] = {0, 0, 0, XI3, XI4, XI5, XI6, XI3, XI3, XI4, XI4, XI5, XI5, XI6, XI6, XI15, XI16, XI17, XI18};
+    const long long cbase = (long long)ks0 * plane + c * 128;
+#pragma unroll
+    for (int q0 = 3; q0 < NQ; q0 += kXiDirs) {
+        // stage levels ks0 .. ks0 + nl - 1 of directions q0 .. q0 + 3 (pairs past the plane skipped)
+#pragma unroll
+        for (int d = 0; d < kXiDirs; ++d) {
+            const double* h = H + (long long)(q0 + d - 3) * a.qs + cbase;
+            for (int t = threadIdx.x; t < nl * 64; t += 64 * kXiLevels) {
+                const int lv = t >> 6, pp = t & 63;
+                if (c * 128 + 2 * pp < plane) S[d][lv][pp] = ld2(h + (long long)lv * plane + 2 * pp);
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int d = 0; d < kXiDirs; ++d) {
+                const double* wz = XI + fam[q0 + d] * 7;
+                d2 v = S[d][sl][lane];
+                double r0 = v.x * wz[0], r1 = v.y * wz[0];
+#pragma unroll
+                for (int m = 1; m < 7; ++m) {
+                    v = S[d][sl + m][lane];
+                    r0 = r0 + v.x * wz[m];
+                    r1 = r1 + v.y * wz[m];
+                }
+                F0[q0 + d] = r0;
+                F1[q0 + d] = r1;
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+#pragma unroll
+    for (int q = 1; q < NQ; ++q) st2(a.fn + (long long)q * a.qs + node, F0[q], F1[q]);
+}
+
 static hipError_t launch_twopass(hipStream_t st, const StepLaunch& L)
 {
     if (L.store != Store::F64 || !L.hbuf) return hipErrorInvalidValue;
@@ -594,14 +685,13 @@ static hipError_t launch_twopass(hipStream_t st, const StepLaunch& L)
     if (L.g.nx % 2 == 0) {  // paired forms: 16-byte taps
         const unsigned nb = (unsigned)twopass_blocks(L.g.nx, L.g.ny, L.g.nz, 2);
         hipLaunchKernelGGL(k_hfilter2, dim3(nb), dim3(256), 0, st, a, (double*)L.hbuf);
-        hipLaunchKernelGGL(k_xi_stream2, dim3(nb), dim3(256), 0, st, a, (const double*)L.hbuf);
+        hipLaunchKernelGGL(k_xi_stream_lds, dim3((unsigned)xi_lds_blocks(L.g.nx, L.g.ny, L.g.nz)), dim3(64 * kXiLevels), 0, st, a,
+                           (const double*)L.hbuf);
     } else {
         const unsigned nb = (unsigned)twopass_blocks(L.g.nx, L.g.ny, L.g.nz);
         hipLaunchKernelGGL(k_hfilter, dim3(nb), dim3(256), 0, st, a, (double*)L.hbuf);
         hipLaunchKernelGGL(k_xi_stream, dim3(nb), dim3(256), 0, st, a, (const double*)L.hbuf);
     }
-    const long long plane = (long long)L.g.nx * L.g.ny;
-    hipLaunchKernelGGL(k_walls_inplace, dim3((unsigned)((plane + 255) / 256), 3), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_collide_inplace, dim3((unsigned)((L.g.n + 255) / 256)), dim3(256), 0, st, a);
     return hipGetLastError();
 }

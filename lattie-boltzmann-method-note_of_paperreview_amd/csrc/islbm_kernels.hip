@@ -678,13 +678,122 @@ __global__ __launch_bounds__(64 * kXiLevels) void k_xi_stream_lds(StepArgs<Store
     for (int q = 1; q < NQ; ++q) st2(a.fn + (long long)q * a.qs + node, F0[q], F1[q]);
 }
 
+// LDS form of the x/y filter pass (even nx): a workgroup takes a tile of 64 pairs x 8 rows of
+// one level.  For each direction the rows and pairs its taps reach (3 rows and 2 pairs of
+// halo, wrapped at load time) are staged once into LDS; 7-10 are filtered in x once per staged
+// row, then in y.  Same sums, same order as the one-node kernel.
+constexpr int kHfRows = 8;
+
+__host__ __device__ inline long long hf_lds_blocks(int nx, int ny, int nz)
+{
+    const long long nb = (long long)((nx / 2 + 63) / 64) * ((ny + kHfRows - 1) / kHfRows) * nz;
+    return 8 * ((nb + 7) / 8);
+}
+
+__global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, double* __restrict__ H)
+{
+    __shared__ d2 S[kHfRows + 6][68];
+    __shared__ d2 X[kHfRows + 6][64];
+    const int nx = a.nx, ny = a.ny, nz = a.nz, np = nx / 2;
+    const int ns = (np + 63) / 64, nr = (ny + kHfRows - 1) / kHfRows;
+    const long long nt = (long long)ns * nr, nb = nt * nz;
+    const long long per = (nb + 7) / 8;
+    const long long b = blockIdx.x;
+    const long long L = (b % 8) * per + b / 8;  // XCD-contiguous run, tiles of one level fastest
+    if (L >= nb) return;                         // whole workgroup: no barrier skipped
+    const int tile = (int)(L % nt), k = (int)(L / nt);
+    const int p0 = (tile % ns) * 64, j0 = (tile / ns) * kHfRows;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int pi = p0 + lane;
+    const bool xin = pi < np;
+    const long long plane = (long long)nx * ny, lev = (long long)k * plane;
+    using cdouble = const __attribute__((address_space(4))) double;
+    const cdouble* X0 = (cdouble*)a.t.xw0;
+    const cdouble* X2 = (cdouble*)a.t.xw2;
+    const cdouble* Y0 = (cdouble*)a.t.yw0;
+    const cdouble* Y2 = (cdouble*)a.t.yw2;
+#pragma unroll
+    for (int q = 3; q < NQ; ++q) {
+        const double* p = a.fo + (long long)q * a.qs + lev;
+        const cdouble* wx = cx(q) > 0 ? X0 : X2;
+        const cdouble* wy = cy(q) > 0 ? Y0 : Y2;
+        const bool hx = cx(q) != 0, hy = cy(q) != 0;
+        const int r_lo = hy ? 0 : 3, nrow = hy ? kHfRows + 6 : kHfRows;
+        const int c_lo = hx ? 0 : 2, ncol = hx ? 68 : 64;
+        for (int t = threadIdx.x; t < nrow * ncol; t += 256) {
+            const int rr = r_lo + t / ncol, pc = c_lo + t % ncol;
+            const int gp = wrap_index(p0 - 2 + pc, np), gj = wrap_index(j0 - 3 + rr, ny);
+            S[rr][pc] = ld2(p + (long long)gj * nx + 2 * gp);
+        }
+        __syncthreads();
+        if (hx && hy) {  // 7-10: x filter of every staged row
+            for (int t = threadIdx.x; t < (kHfRows + 6) * 64; t += 256) {
+                const int rr = t >> 6, pc = t & 63;
+                double w[10];
+#pragma unroll
+                for (int m = 0; m < 5; ++m) {
+                    const d2 v = S[rr][pc + m];
+                    w[2 * m] = v.x;
+                    w[2 * m + 1] = v.y;
+                }
+                double r0 = w[1] * wx[0], r1 = w[2] * wx[0];
+#pragma unroll
+                for (int l = 1; l < 7; ++l) {
+                    r0 = r0 + w[1 + l] * wx[l];
+                    r1 = r1 + w[2 + l] * wx[l];
+                }
+                X[rr][pc] = d2(r0, r1);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int rl = 0; rl < kHfRows / 4; ++rl) {
+            const int r = wv * (kHfRows / 4) + rl, j = j0 + r;
+            if (j >= ny || !xin) continue;
+            double r0, r1;
+            if (!hx && !hy) {  // 5, 6: at the nodes
+                const d2 v = S[r + 3][lane + 2];
+                r0 = v.x;
+                r1 = v.y;
+            } else if (!hy) {  // 11-14: x only
+                double w[10];
+#pragma unroll
+                for (int m = 0; m < 5; ++m) {
+                    const d2 v = S[r + 3][lane + m];
+                    w[2 * m] = v.x;
+                    w[2 * m + 1] = v.y;
+                }
+                r0 = w[1] * wx[0];
+                r1 = w[2] * wx[0];
+#pragma unroll
+                for (int l = 1; l < 7; ++l) {
+                    r0 = r0 + w[1 + l] * wx[l];
+                    r1 = r1 + w[2 + l] * wx[l];
+                }
+            } else {  // y only (3, 4, 15-18) or y of the x-filtered rows (7-10)
+                d2 v = hx ? X[r][lane] : S[r][lane + 2];
+                r0 = v.x * wy[0];
+                r1 = v.y * wy[0];
+#pragma unroll
+                for (int n = 1; n < 7; ++n) {
+                    v = hx ? X[r + n][lane] : S[r + n][lane + 2];
+                    r0 = r0 + v.x * wy[n];
+                    r1 = r1 + v.y * wy[n];
+                }
+            }
+            st2(H + (long long)(q - 3) * a.qs + lev + (long long)j * nx + 2 * pi, r0, r1);
+        }
+        __syncthreads();
+    }
+}
+
 static hipError_t launch_twopass(hipStream_t st, const StepLaunch& L)
 {
     if (L.store != Store::F64 || !L.hbuf) return hipErrorInvalidValue;
     StepArgs<Store::F64> a = make_step_args<Store::F64>(L);
     if (L.g.nx % 2 == 0) {  // paired forms: 16-byte taps
-        const unsigned nb = (unsigned)twopass_blocks(L.g.nx, L.g.ny, L.g.nz, 2);
-        hipLaunchKernelGGL(k_hfilter2, dim3(nb), dim3(256), 0, st, a, (double*)L.hbuf);
+        hipLaunchKernelGGL(k_hfilter_lds, dim3((unsigned)hf_lds_blocks(L.g.nx, L.g.ny, L.g.nz)), dim3(256), 0, st, a,
+                           (double*)L.hbuf);
         hipLaunchKernelGGL(k_xi_stream_lds, dim3((unsigned)xi_lds_blocks(L.g.nx, L.g.ny, L.g.nz)), dim3(64 * kXiLevels), 0, st, a,
                            (const double*)L.hbuf);
     } else {

@@ -257,8 +257,8 @@ def test_fp64_auto_is_twopass_and_equals_pernode(pkg, nx):
     """FP64 solvers take the two-pass kernel by default; it sums the same taps as the
     per-node kernel in another order (3 and 4 filtered before the xi sum), so the two agree
     to FP64 round-off after many steps, at shapes with partial workgroups (paired forms at
-    even nx, one-node forms at odd nx)."""
-    kw = dict(nx=nx, ny=48, nz=36, tau=0.6833, re=50.0, mesh_mode=0, u_ref=0.0, precision="fp64", cs_smag=0.1)
+    even nx, one-node forms at odd nx; ny = 44 leaves a partial band of rows)."""
+    kw = dict(nx=nx, ny=44, nz=36, tau=0.6833, re=50.0, mesh_mode=0, u_ref=0.0, precision="fp64", cs_smag=0.1)
     a = pkg.Solver(pkg.default_params(**kw))
     b = pkg.Solver(pkg.default_params(kernel=pkg.KERNEL_PERNODE, **kw))
     assert a.kernel_in_use() == pkg.KERNEL_TWOPASS

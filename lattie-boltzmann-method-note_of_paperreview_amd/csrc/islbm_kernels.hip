@@ -576,10 +576,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // LDS form of the xi pass (even nx): a workgroup takes one chunk of 128 nodes (64 pairs) of
 // the (i, j) plane at 8 consecutive levels, one wave per level.  For each group of 4
 // directions the levels the 8 stencils share (at most 14) are staged once into LDS, so each
-// value of H is requested from L2 about 1.75 times instead of 7.
+// value of H is requested from L2 about 1.75 times instead of 7.  The rounds are pipelined:
+// the next round's values are loaded into registers while the current one is summed.
 constexpr int kXiLevels = 8;    // levels per workgroup (one wave each)
 constexpr int kXiStage = 14;    // staged levels: 8 + 6
-constexpr int kXiDirs = 4;      // directions per staging round
+constexpr int kXiDirs = 2;      // directions per staging round (two rounds in LDS at a time)
 
 __host__ __device__ inline long long xi_lds_blocks(int nx, int ny, int nz)
 {
@@ -588,9 +589,9 @@ __host__ __device__ inline long long xi_lds_blocks(int nx, int ny, int nz)
     return 8 * ((nch * ng + 7) / 8);
 }
 
-__global__ __launch_bounds__(64 * kXiLevels) void k_xi_stream_lds(StepArgs<Store::F64> a, const double* __restrict__ H)
+__global__ __launch_bounds__(64 * kXiLevels) __attribute__((amdgpu_waves_per_eu(4))) void k_xi_stream_lds(StepArgs<Store::F64> a, const double* __restrict__ H)
 {
-    __shared__ d2 S[kXiDirs][kXiStage][64];
+    __shared__ d2 S[2][kXiDirs][kXiStage][64];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long long plane = (long long)nx * ny;
     const long long nch = (plane + 127) / 128;
@@ -614,7 +615,6 @@ __global__ __launch_bounds__(64 * kXiLevels) void k_xi_stream_lds(StepArgs<Store
     const double* XI = a.t.xiw + ((long long)kk * ny + (active ? j : 0)) * (NXI * 7);
     const long long row = (long long)kk * plane + (long long)j * nx;
     const long long node = row + i;
-    double F0[NQ], F1[NQ];
     if (active) {  // directions 1 and 2: x filters of the old populations at the node's row
         int xp[5];
 #pragma unroll
@@ -637,45 +637,59 @@ __global__ __launch_bounds__(64 * kXiLevels) void k_xi_stream_lds(StepArgs<Store
                 r0 = r0 + w[1 + l] * wx[l];
                 r1 = r1 + w[2 + l] * wx[l];
             }
-            F0[q] = r0;
-            F1[q] = r1;
+            st2(a.fn + (long long)q * a.qs + node, r0, r1);
         }
     }
     constexpr int fam[NQ] = {0, 0, 0, XI3, XI4, XI5, XI6, XI3, XI3, XI4, XI4, XI5, XI5, XI6, XI6, XI15, XI16, XI17, XI18};
     const long long cbase = (long long)ks0 * plane + c * 128;
-#pragma unroll
-    for (int q0 = 3; q0 < NQ; q0 += kXiDirs) {
-        // stage levels ks0 .. ks0 + nl - 1 of directions q0 .. q0 + 3 (pairs past the plane skipped)
+    constexpr int kPer = (kXiStage * 64 + 64 * kXiLevels - 1) / (64 * kXiLevels);  // staged pairs per thread
+    d2 R[kXiDirs][kPer];
+    // levels ks0 .. ks0 + nl - 1 of directions q0 .. q0 + kXiDirs - 1 (pairs past the plane skipped)
+    auto fetch = [&](int q0) {
 #pragma unroll
         for (int d = 0; d < kXiDirs; ++d) {
             const double* h = H + (long long)(q0 + d - 3) * a.qs + cbase;
-            for (int t = threadIdx.x; t < nl * 64; t += 64 * kXiLevels) {
-                const int lv = t >> 6, pp = t & 63;
-                if (c * 128 + 2 * pp < plane) S[d][lv][pp] = ld2(h + (long long)lv * plane + 2 * pp);
+#pragma unroll
+            for (int it = 0; it < kPer; ++it) {
+                const int t = threadIdx.x + it * 64 * kXiLevels, lv = t >> 6, pp = t & 63;
+                if (t < nl * 64 && c * 128 + 2 * pp < plane) R[d][it] = ld2(h + (long long)lv * plane + 2 * pp);
             }
         }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int d = 0; d < kXiDirs; ++d)
+#pragma unroll
+            for (int it = 0; it < kPer; ++it) {
+                const int t = threadIdx.x + it * 64 * kXiLevels;
+                if (t < nl * 64) S[buf][d][t >> 6][t & 63] = R[d][it];
+            }
+    };
+    fetch(3);
+#pragma unroll
+    for (int q0 = 3; q0 < NQ; q0 += kXiDirs) {
+        const int buf = ((q0 - 3) / kXiDirs) & 1;
+        put(buf);
+        // one barrier per round: a wave past it has finished the sums of the round before,
+        // whose buffer the next round's put overwrites
         __syncthreads();
+        if (q0 + kXiDirs < NQ) fetch(q0 + kXiDirs);
         if (active) {
 #pragma unroll
             for (int d = 0; d < kXiDirs; ++d) {
                 const double* wz = XI + fam[q0 + d] * 7;
-                d2 v = S[d][sl][lane];
+                d2 v = S[buf][d][sl][lane];
                 double r0 = v.x * wz[0], r1 = v.y * wz[0];
 #pragma unroll
                 for (int m = 1; m < 7; ++m) {
-                    v = S[d][sl + m][lane];
+                    v = S[buf][d][sl + m][lane];
                     r0 = r0 + v.x * wz[m];
                     r1 = r1 + v.y * wz[m];
                 }
-                F0[q0 + d] = r0;
-                F1[q0 + d] = r1;
+                st2(a.fn + (long long)(q0 + d) * a.qs + node, r0, r1);  // streamed populations
             }
         }
-        __syncthreads();
     }
-    if (!active) return;
-#pragma unroll
-    for (int q = 1; q < NQ; ++q) st2(a.fn + (long long)q * a.qs + node, F0[q], F1[q]);
 }
 
 // LDS form of the x/y filter pass (even nx): a workgroup takes a tile of 64 pairs x 8 rows of

@@ -695,7 +695,8 @@ __global__ __launch_bounds__(64 * kXiLevels) __attribute__((amdgpu_waves_per_eu(
 // LDS form of the x/y filter pass (even nx): a workgroup takes a tile of 64 pairs x 8 rows of
 // one level.  For each direction the rows and pairs its taps reach (3 rows and 2 pairs of
 // halo, wrapped at load time) are staged once into LDS; 7-10 are filtered in x once per staged
-// row, then in y.  Same sums, same order as the one-node kernel.
+// row, then in y.  The next direction's values are loaded into registers while the current
+// one is filtered.  Same sums, same order as the one-node kernel.
 constexpr int kHfRows = 8;
 
 __host__ __device__ inline long long hf_lds_blocks(int nx, int ny, int nz)
@@ -726,20 +727,43 @@ __global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, dou
     const cdouble* X2 = (cdouble*)a.t.xw2;
     const cdouble* Y0 = (cdouble*)a.t.yw0;
     const cdouble* Y2 = (cdouble*)a.t.yw2;
-#pragma unroll
-    for (int q = 3; q < NQ; ++q) {
+    // the staged rows and pairs of direction q: rows r_lo .. r_lo + nrow - 1, pairs c_lo .. c_lo + ncol - 1
+    constexpr int kPer = ((kHfRows + 6) * 68 + 255) / 256;
+    d2 R[kPer];
+    auto fetch = [&](int q) {
         const double* p = a.fo + (long long)q * a.qs + lev;
-        const cdouble* wx = cx(q) > 0 ? X0 : X2;
-        const cdouble* wy = cy(q) > 0 ? Y0 : Y2;
         const bool hx = cx(q) != 0, hy = cy(q) != 0;
         const int r_lo = hy ? 0 : 3, nrow = hy ? kHfRows + 6 : kHfRows;
         const int c_lo = hx ? 0 : 2, ncol = hx ? 68 : 64;
-        for (int t = threadIdx.x; t < nrow * ncol; t += 256) {
-            const int rr = r_lo + t / ncol, pc = c_lo + t % ncol;
-            const int gp = wrap_index(p0 - 2 + pc, np), gj = wrap_index(j0 - 3 + rr, ny);
-            S[rr][pc] = ld2(p + (long long)gj * nx + 2 * gp);
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int t = threadIdx.x + it * 256;
+            if (t < nrow * ncol) {
+                const int rr = r_lo + t / ncol, pc = c_lo + t % ncol;
+                const int gp = wrap_index(p0 - 2 + pc, np), gj = wrap_index(j0 - 3 + rr, ny);
+                R[it] = ld2(p + (long long)gj * nx + 2 * gp);
+            }
         }
+    };
+    auto put = [&](int q) {
+        const bool hx = cx(q) != 0, hy = cy(q) != 0;
+        const int r_lo = hy ? 0 : 3, nrow = hy ? kHfRows + 6 : kHfRows;
+        const int c_lo = hx ? 0 : 2, ncol = hx ? 68 : 64;
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int t = threadIdx.x + it * 256;
+            if (t < nrow * ncol) S[r_lo + t / ncol][c_lo + t % ncol] = R[it];
+        }
+    };
+    fetch(3);
+#pragma unroll
+    for (int q = 3; q < NQ; ++q) {
+        const cdouble* wx = cx(q) > 0 ? X0 : X2;
+        const cdouble* wy = cy(q) > 0 ? Y0 : Y2;
+        const bool hx = cx(q) != 0, hy = cy(q) != 0;
+        put(q);
         __syncthreads();
+        if (q + 1 < NQ) fetch(q + 1);  // in flight while this direction is filtered
         if (hx && hy) {  // 7-10: x filter of every staged row
             for (int t = threadIdx.x; t < (kHfRows + 6) * 64; t += 256) {
                 const int rr = t >> 6, pc = t & 63;

@@ -304,6 +304,22 @@ def prod_fp64(pkg):
     return m
 
 
+def test_fp64_twopass_at_production_shape(pkg, prod_fp64):
+    """The FP64 default (two-pass) kernel against the per-node FP64 kernel at 128 x 256 x 128
+    after the same 200 LES steps: the same sums in another order, FP64 round-off apart."""
+    s = pkg.Solver(pkg.default_params(**PROD, precision="fp64"))
+    assert s.kernel_in_use() == pkg.KERNEL_TWOPASS
+    s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    s.step(200)
+    r, u, v, w = s.macros()
+    s.close()
+    ro, uo, vo, wo = prod_fp64
+    e_u = float(np.sqrt(np.sum((u - uo) ** 2 + (v - vo) ** 2 + (w - wo) ** 2) / np.sum(uo ** 2 + vo ** 2 + wo ** 2)))
+    e_rho = float(np.linalg.norm(r - ro) / np.linalg.norm(ro))
+    print(f"fp64 two-pass vs per-node 128x256x128 200 steps: rel L2 u {e_u:.3e} rho {e_rho:.3e}", flush=True)
+    assert e_u <= 1e-12 and e_rho <= 1e-13, (e_u, e_rho)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("fp16", 6e-3), ("bf16", 2e-2)])
 def test_velocity_error_at_production_shape(pkg, prod_fp64, prec, tol):
     """Relative L2 velocity error of the tiled kernel against FP64 at 128 x 256 x 128 after

@@ -574,13 +574,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // LDS form of the xi pass (even nx): a workgroup takes one chunk of 128 nodes (64 pairs) of
-// the (i, j) plane at 16 consecutive levels, one wave per level.  For each group of 2
-// directions the levels the 16 stencils share (at most 22) are staged once into LDS, so each
-// value of H is requested from L2 about 1.4 times instead of 7.  The rounds are pipelined:
+// the (i, j) plane at 8 consecutive levels, one wave per level.  For each direction the
+// levels the 8 stencils share (at most 14) are staged once into LDS, so each value of H is
+// requested from L2 about 1.75 times instead of 7; 28 KB of LDS lets several workgroups share
+// a CU and cover each other's barriers.  The rounds are pipelined:
 // the next round's values are loaded into registers while the current one is summed.
-constexpr int kXiLevels = 16;   // levels per workgroup (one wave each)
-constexpr int kXiStage = 22;    // staged levels: 16 + 6
-constexpr int kXiDirs = 2;      // directions per staging round (two rounds in LDS at a time)
+constexpr int kXiLevels = 8;    // levels per workgroup (one wave each)
+constexpr int kXiStage = 14;    // staged levels: 8 + 6
+constexpr int kXiDirs = 1;      // directions per staging round (two rounds in LDS at a time)
 
 __host__ __device__ inline long long xi_lds_blocks(int nx, int ny, int nz)
 {

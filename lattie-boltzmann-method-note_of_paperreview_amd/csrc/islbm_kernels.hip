@@ -693,12 +693,12 @@ __global__ __launch_bounds__(64 * kXiLevels) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
-// LDS form of the x/y filter pass (even nx): a workgroup takes a tile of 64 pairs x 8 rows of
+// LDS form of the x/y filter pass (even nx): a workgroup takes a tile of 64 pairs x 16 rows of
 // one level.  For each direction the rows and pairs its taps reach (3 rows and 2 pairs of
 // halo, wrapped at load time) are staged once into LDS; 7-10 are filtered in x once per staged
 // row, then in y.  The next direction's values are loaded into registers while the current
 // one is filtered.  Same sums, same order as the one-node kernel.
-constexpr int kHfRows = 8;
+constexpr int kHfRows = 16;
 
 __host__ __device__ inline long long hf_lds_blocks(int nx, int ny, int nz)
 {

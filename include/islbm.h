@@ -49,7 +49,10 @@ extern "C" {
 #define ISLBM_KERNEL_AUTO 0      /* FP64: two-pass; else tiled where supported, else per-node */
 #define ISLBM_KERNEL_PERNODE 1   /* one thread per node, direct neighbour reads          */
 #define ISLBM_KERNEL_TILED 2     /* column-marching, LDS-staged, sum-factorised          */
-#define ISLBM_KERNEL_TWOPASS 3   /* FP64: x/y filter pass, then xi + walls + collision   */
+#define ISLBM_KERNEL_TWOPASS 3   /* FP64 only: x/y filter pass into a 16-direction buffer  */
+                                 /* (16 doubles per node, counted by islbm_required_bytes), */
+                                 /* then the xi sums, then walls + collision in place;      */
+                                 /* ISLBM_ERR_UNSUPPORTED for the other precisions          */
 
 /* Run-time form of the reference's compile-time parameter set (variables.h:5-102). */
 typedef struct islbm_params {

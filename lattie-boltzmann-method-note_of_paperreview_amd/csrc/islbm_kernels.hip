@@ -699,6 +699,7 @@ __global__ __launch_bounds__(64 * kXiLevels) __attribute__((amdgpu_waves_per_eu(
 // row, then in y.  The next direction's values are loaded into registers while the current
 // one is filtered.  Same sums, same order as the one-node kernel.
 constexpr int kHfRows = 16;
+constexpr int kHfThreads = 512;  // 8 waves, 2 rows each
 
 __host__ __device__ inline long long hf_lds_blocks(int nx, int ny, int nz)
 {
@@ -706,7 +707,7 @@ __host__ __device__ inline long long hf_lds_blocks(int nx, int ny, int nz)
     return 8 * ((nb + 7) / 8);
 }
 
-__global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, double* __restrict__ H)
+__global__ __launch_bounds__(kHfThreads) void k_hfilter_lds(StepArgs<Store::F64> a, double* __restrict__ H)
 {
     __shared__ d2 S[kHfRows + 6][68];
     __shared__ d2 X[kHfRows + 6][64];
@@ -729,7 +730,7 @@ __global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, dou
     const cdouble* Y0 = (cdouble*)a.t.yw0;
     const cdouble* Y2 = (cdouble*)a.t.yw2;
     // the staged rows and pairs of direction q: rows r_lo .. r_lo + nrow - 1, pairs c_lo .. c_lo + ncol - 1
-    constexpr int kPer = ((kHfRows + 6) * 68 + 255) / 256;
+    constexpr int kPer = ((kHfRows + 6) * 68 + kHfThreads - 1) / kHfThreads;
     d2 R[kPer];
     auto fetch = [&](int q) {
         const double* p = a.fo + (long long)q * a.qs + lev;
@@ -738,7 +739,7 @@ __global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, dou
         const int c_lo = hx ? 0 : 2, ncol = hx ? 68 : 64;
 #pragma unroll
         for (int it = 0; it < kPer; ++it) {
-            const int t = threadIdx.x + it * 256;
+            const int t = threadIdx.x + it * kHfThreads;
             if (t < nrow * ncol) {
                 const int rr = r_lo + t / ncol, pc = c_lo + t % ncol;
                 const int gp = wrap_index(p0 - 2 + pc, np), gj = wrap_index(j0 - 3 + rr, ny);
@@ -752,7 +753,7 @@ __global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, dou
         const int c_lo = hx ? 0 : 2, ncol = hx ? 68 : 64;
 #pragma unroll
         for (int it = 0; it < kPer; ++it) {
-            const int t = threadIdx.x + it * 256;
+            const int t = threadIdx.x + it * kHfThreads;
             if (t < nrow * ncol) S[r_lo + t / ncol][c_lo + t % ncol] = R[it];
         }
     };
@@ -766,7 +767,7 @@ __global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, dou
         __syncthreads();
         if (q + 1 < NQ) fetch(q + 1);  // in flight while this direction is filtered
         if (hx && hy) {  // 7-10: x filter of every staged row
-            for (int t = threadIdx.x; t < (kHfRows + 6) * 64; t += 256) {
+            for (int t = threadIdx.x; t < (kHfRows + 6) * 64; t += kHfThreads) {
                 const int rr = t >> 6, pc = t & 63;
                 double w[10];
 #pragma unroll
@@ -786,8 +787,8 @@ __global__ __launch_bounds__(256) void k_hfilter_lds(StepArgs<Store::F64> a, dou
             __syncthreads();
         }
 #pragma unroll
-        for (int rl = 0; rl < kHfRows / 4; ++rl) {
-            const int r = wv * (kHfRows / 4) + rl, j = j0 + r;
+        for (int rl = 0; rl < kHfRows / (kHfThreads / 64); ++rl) {
+            const int r = wv * (kHfRows / (kHfThreads / 64)) + rl, j = j0 + r;
             if (j >= ny || !xin) continue;
             double r0, r1;
             if (!hx && !hy) {  // 5, 6: at the nodes
@@ -831,7 +832,7 @@ static hipError_t launch_twopass(hipStream_t st, const StepLaunch& L)
     if (L.store != Store::F64 || !L.hbuf) return hipErrorInvalidValue;
     StepArgs<Store::F64> a = make_step_args<Store::F64>(L);
     if (L.g.nx % 2 == 0) {  // paired forms: 16-byte taps
-        hipLaunchKernelGGL(k_hfilter_lds, dim3((unsigned)hf_lds_blocks(L.g.nx, L.g.ny, L.g.nz)), dim3(256), 0, st, a,
+        hipLaunchKernelGGL(k_hfilter_lds, dim3((unsigned)hf_lds_blocks(L.g.nx, L.g.ny, L.g.nz)), dim3(kHfThreads), 0, st, a,
                            (double*)L.hbuf);
         hipLaunchKernelGGL(k_xi_stream_lds, dim3((unsigned)xi_lds_blocks(L.g.nx, L.g.ny, L.g.nz)), dim3(64 * kXiLevels), 0, st, a,
                            (const double*)L.hbuf);

@@ -173,7 +173,7 @@ def main() -> int:
                    "state_finite": finite, "event_ms_per_step": round(total_ms / args.steps, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_hfilter+k_xi_collide" if ktag == "twopass" else f"k_collide_{ktag}", "bytes_per_lup": bpl,
+                     "kernel": "k_hfilter_lds+k_xi_stream_lds+k_collide_inplace" if ktag == "twopass" else f"k_collide_{ktag}", "bytes_per_lup": bpl,
                      "avg_launch_us": round(avg_collide_s * 1e6, 2)},
     }
     s.close()

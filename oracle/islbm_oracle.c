@@ -647,8 +647,9 @@ static double les_tau(const double F[NQ], const double Feq[NQ], double rho, doub
     return tau0 + 3.0 * C * S / dt;
 }
 
-/* One node of stream_collide: evolution.h:398-683. */
-static void node_update(or_state* s, double* const* fo, double* const* fn, int I, int J, int K)
+/* Streaming half of stream_collide at one node: interpolation, flat-wall bounce-back and
+ * BFL (evolution.h:451-572).  F receives the populations that enter the collision. */
+static void stream_node(const or_state* s, double* const* fo, int I, int J, int K, double F[NQ])
 {
     const int NX6 = s->NX6, NY6 = s->NY6, NZ6 = s->NZ6;
     const long nface = s->nface, nline = NX6;
@@ -658,7 +659,6 @@ static void node_update(or_state* s, double* const* fo, double* const* fn, int I
     int cz = K - 3;
     if (K <= 6) cz = 3;
     if (K >= NZ6 - 7) cz = NZ6 - 10;
-    double F[NQ];
 
     /* interpolated streaming: evolution.h:451-475 */
     F[0] = fo[0][index];
@@ -738,6 +738,14 @@ static void node_update(or_state* s, double* const* fo, double* const* fn, int I
         }
     }
 
+}
+
+/* Collision half of stream_collide at one node (evolution.h:597-683): mass fix, moments,
+ * equilibrium, MRT (+ Smagorinsky) with the body force.  F is updated in place to the
+ * post-collision populations; mac = [rho, u, v, w, tau_eff]. */
+static void collide_node(const or_state* s, double F[NQ], int J, int K, double mac[5])
+{
+    const int NZ6 = s->NZ6;
     /* mass fix and macroscopic quantities: evolution.h:597-606 */
     F[0] = F[0] + s->rho_modify;
     double rho_s = F[0];
@@ -779,12 +787,27 @@ static void node_update(or_state* s, double* const* fo, double* const* fn, int I
             const double c = (E[q][0] == 0 && E[q][2] == 0) ? 1.0 / 6 : 1.0 / 12;
             r = (E[q][1] > 0) ? r + c * fy * s->dt : r - c * fy * s->dt;
         }
-        fn[q][index] = r;
+        F[q] = r;
     }
-    s->rho[index] = rho_s;
-    s->u[index] = u1;
-    s->v[index] = v1;
-    s->w[index] = w1;
+    mac[0] = rho_s;
+    mac[1] = u1;
+    mac[2] = v1;
+    mac[3] = w1;
+    mac[4] = tau_eff;
+}
+
+/* One node of stream_collide: evolution.h:398-683. */
+static void node_update(or_state* s, double* const* fo, double* const* fn, int I, int J, int K)
+{
+    const long index = (long)J * s->nface + (long)K * s->NX6 + I;
+    double F[NQ], mac[5];
+    stream_node(s, fo, I, J, K, F);
+    collide_node(s, F, J, K, mac);
+    for (int q = 0; q < NQ; ++q) fn[q][index] = F[q];
+    s->rho[index] = mac[0];
+    s->u[index] = mac[1];
+    s->v[index] = mac[2];
+    s->w[index] = mac[3];
 }
 
 /* periodicSW (evolution.h:732-777) and the single-rank y halo (communication.h:11-82 with
@@ -856,6 +879,39 @@ void or_step(or_state* s, int nsteps)
         s->step++;
         if (s->p.ndtfrc > 0 && s->step % s->p.ndtfrc == 0) or_modify_forcing(s);
     }
+}
+
+/* Test entry (no reference counterpart): one step of the streaming half alone.  The
+ * populations that would enter the collision (interpolated streaming, bounce-back and BFL)
+ * become the current populations; macroscopic fields, the flux probe, the force and the
+ * step count are left as they are. */
+void or_stream_only(or_state* s)
+{
+    double* const* fo = s->f[s->cur];
+    double* const* fn = s->f[1 - s->cur];
+    const int NX6 = s->NX6, NY6 = s->NY6, NZ6 = s->NZ6;
+#pragma omp parallel for schedule(static)
+    for (int J = 3; J < NY6 - 3; ++J)
+        for (int K = 3; K < NZ6 - 3; ++K)
+            for (int I = 3; I < NX6 - 3; ++I) {
+                double F[NQ];
+                stream_node(s, fo, I, J, K, F);
+                const long index = (long)J * s->nface + (long)K * NX6 + I;
+                for (int q = 0; q < NQ; ++q) fn[q][index] = F[q];
+            }
+    fill_ghosts(s, fn);
+    s->cur = 1 - s->cur;
+}
+
+/* Test entry: the collision half at unique node (j, k) on given populations f_in[19]
+ * (the node's position sets the Smagorinsky filter width).  f_out receives the
+ * post-collision populations, mac = [rho, u, v, w, tau_eff]. */
+void or_collide_one(const or_state* s, const double* f_in, int j, int k, double* f_out, double* mac)
+{
+    double F[NQ];
+    for (int q = 0; q < NQ; ++q) F[q] = f_in[q];
+    collide_node(s, F, j + 3, k + 3, mac);
+    for (int q = 0; q < NQ; ++q) f_out[q] = F[q];
 }
 
 double or_get_force(const or_state* s) { return s->force; }

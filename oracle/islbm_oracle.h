@@ -62,6 +62,14 @@ void      or_set_rho_modify(or_state* s, double dr);
 double    or_flux_sum(const or_state* s);    /* current sum of the Ub accumulator */
 long      or_step_count(const or_state* s);
 
+/* Test entries that split the step (no reference counterpart):
+ *   or_stream_only: interpolated streaming + bounce-back + BFL of every node, without the
+ *                   collision, into the current populations;
+ *   or_collide_one: the collision of one node (j, k) on given f_in[19] -> f_out[19],
+ *                   mac[5] = [rho, u, v, w, tau_eff]. */
+void      or_stream_only(or_state* s);
+void      or_collide_one(const or_state* s, const double* f_in, int j, int k, double* f_out, double* mac);
+
 /* Geometry tables, unique-node form (see islbm_oracle.c for layouts). */
 int       or_get_table(const or_state* s, const char* name, double* out, int cap);
 

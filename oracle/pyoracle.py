@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libislbm_oracle.so")
+# ISLBM_ORACLE_LIB names another build of the oracle (the mutation tests load planted-mistake
+# builds through it)
+LIB_PATH = os.environ.get("ISLBM_ORACLE_LIB") or os.path.join(HERE, "_build", "libislbm_oracle.so")
 REF_GEOM = os.path.join(HERE, "_ref", "ref_geom")
 
 
@@ -86,6 +88,8 @@ def lib():
         L.or_stats_get.argtypes = [vp, ctypes.c_char_p, dp]
         L.or_stats_get.restype = ip
         L.or_set_macros.argtypes = [vp, dp, dp, dp, dp]
+        L.or_stream_only.argtypes = [vp]
+        L.or_collide_one.argtypes = [vp, dp, ip, ip, dp, dp]
         _lib = L
     return _lib
 
@@ -170,6 +174,18 @@ class Oracle:
 
     def step(self, n: int):
         lib().or_step(self.h, int(n))
+
+    def stream_only(self):
+        """Interpolated streaming + bounce-back + BFL of one step, no collision (test entry)."""
+        lib().or_stream_only(self.h)
+
+    def collide_one(self, f_in, j: int, k: int):
+        """Collision of one node (j, k) on f_in[19]: (f_out[19], [rho, u, v, w, tau_eff])."""
+        fi = np.ascontiguousarray(f_in, dtype=np.float64)
+        fo = np.zeros(19)
+        mac = np.zeros(5)
+        lib().or_collide_one(self.h, _dp(fi), int(j), int(k), _dp(fo), _dp(mac))
+        return fo, mac
 
     def modify_forcing(self):
         lib().or_modify_forcing(self.h)

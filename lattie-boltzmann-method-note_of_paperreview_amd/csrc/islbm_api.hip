@@ -149,8 +149,15 @@ struct islbm_solver {
     bool auto_force = true;  // run the forcing controller every ndtfrc steps
     size_t bytes = 0;
     std::vector<hipEvent_t> evpool;
-    hipGraphExec_t graph = nullptr;  // islbm_step_graph: the captured chunk
-    int graph_steps = 0;
+    // islbm_step_graph: the captured chunk per starting buffer parity (a graph's launches
+    // name the population buffers of the parity it was recorded at)
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
+    // bulk-flux probe fused into the tiled kernel: per-step partials by step parity and the
+    // probe weights dx [nx] followed by dz [nz]
+    double* flux_part = nullptr;
+    double* flux_dxz = nullptr;
+    int flux_nslot = 0;
     double* red = nullptr;           // kSumBlocks partials + the sum (mass correction)
     // turbulence statistics (islbm_stats_enable)
     double* st_acc = nullptr;        // [NSTAT][n] FP64 sums
@@ -168,8 +175,9 @@ void free_all(islbm_solver* s)
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
-    if (s->graph) (void)hipGraphExecDestroy(s->graph);
-    for (void* q : {(void*)s->red, (void*)s->st_acc, (void*)s->st_zdw})
+    for (hipGraphExec_t g : s->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (void* q : {(void*)s->red, (void*)s->st_acc, (void*)s->st_zdw, (void*)s->flux_part, (void*)s->flux_dxz})
         if (q) (void)hipFree(q);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -335,7 +343,25 @@ StepLaunch make_launch(islbm_solver* s)
     L.dt = s->g.dt;
     L.cs_smag = s->p.cs_smag;
     L.scale = (s->store == Store::BF16 || s->store == Store::FP16) ? s->scale : nullptr;
+    L.flux_part = s->flux_part;
+    L.flux_dx = s->flux_dxz;
+    L.flux_dz = s->flux_dxz ? s->flux_dxz + s->p.nx : nullptr;
+    L.flux_acc = s->scal + 2;
+    L.flux_plane = s->g.flux_plane;
+    L.flux_par = s->cur;
+    L.flux_nslot = s->flux_nslot;
     return L;
+}
+
+// After v was set from outside the step: the probe partials the next fused step adds are
+// those of the new v (the reference's probe reads whatever v holds before the step).
+int flux_refresh(islbm_solver* s)
+{
+    if (!s->flux_part) return ISLBM_OK;
+    HIPCHK(launch_flux_refresh(s->stream, s->mac[2], s->fp64, s->flux_dxz, s->flux_dxz + s->p.nx,
+                               s->flux_part + (size_t)(1 - s->cur) * s->flux_nslot, s->flux_nslot, s->p.nx, s->p.ny,
+                               s->p.nz, s->g.flux_plane));
+    return ISLBM_OK;
 }
 
 int enqueue_force_update(islbm_solver* s)
@@ -349,8 +375,9 @@ int enqueue_force_update(islbm_solver* s)
 // one Launch_CollisionStreaming; ev (optional) brackets the stream-collide launch
 int enqueue_step(islbm_solver* s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    HIPCHK(launch_flux_probe(s->stream, s->mac[2], s->fp64, s->fluxw, s->fluxw + (size_t)s->p.nx * s->p.nz, s->scal + 2,
-                             s->p.nx, s->p.ny, s->p.nz, s->g.flux_plane));
+    if (!s->flux_part)  // the tiled kernel probes the flux itself
+        HIPCHK(launch_flux_probe(s->stream, s->mac[2], s->fp64, s->fluxw, s->fluxw + (size_t)s->p.nx * s->p.nz,
+                                 s->scal + 2, s->p.nx, s->p.ny, s->p.nz, s->g.flux_plane));
     if (ev0) HIPCHK(hipEventRecord(ev0, s->stream));
     StepLaunch L = make_launch(s);
     HIPCHK(launch_stream_collide(s->stream, L));
@@ -467,6 +494,16 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
     if (hipMemcpy(s->scal, sc, sizeof sc, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(ISLBM_ERR_HIP, "scalars"));
     int rc = s->fp64 ? upload_tables<double>(s) : upload_tables<float>(s);
     if (rc) return bail(rc);
+    if (s->kernel == ISLBM_KERNEL_TILED) {
+        s->flux_nslot = tiled_flux_slots(lg, s->store);
+        const size_t pb = 2 * (size_t)s->flux_nslot * sizeof(double);
+        std::vector<double> dxz(s->g.flux_dx);
+        dxz.insert(dxz.end(), s->g.flux_dz.begin(), s->g.flux_dz.end());
+        if (hipMalloc((void**)&s->flux_part, pb) != hipSuccess || hipMemset(s->flux_part, 0, pb) != hipSuccess ||
+            hipMalloc((void**)&s->flux_dxz, dxz.size() * sizeof(double)) != hipSuccess ||
+            hipMemcpy(s->flux_dxz, dxz.data(), dxz.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(ISLBM_ERR_HIP, "flux probe buffers"));
+    }
     *out = s;
     return ISLBM_OK;
 }
@@ -491,6 +528,7 @@ int islbm_init_equilibrium(islbm_solver* s, double rho0, const double u0[3], dou
                                    rho0, u0, amp, seed, s->scale));
     const double sc[3] = {s->g.force0, 0.0, 0.0};
     HIPCHK(hipMemcpyAsync(s->scal, sc, sizeof sc, hipMemcpyHostToDevice, s->stream));
+    if (int rc = flux_refresh(s)) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     s->steps = 0;
     return ISLBM_OK;
@@ -550,14 +588,16 @@ int islbm_step_graph(islbm_solver* s, int32_t nsteps)
     // statistics cadence: step eagerly
     if (s->auto_force && (s->steps + nsteps) / s->p.ndtfrc > s->steps / s->p.ndtfrc) return islbm_step(s, nsteps);
     if (s->st_acc && s->st_every > 0) return islbm_step(s, nsteps);
-    if (!s->graph || s->graph_steps != nsteps) {
-        if (s->graph) {
-            HIPCHK(hipGraphExecDestroy(s->graph));
-            s->graph = nullptr;
-            s->graph_steps = 0;
+    // the chunk recorded from this buffer parity (after an even number of steps the parity
+    // returns to cur, so a graph is replayed only from the parity it was recorded at)
+    hipGraphExec_t& gx = s->graph[s->cur];
+    if (!gx || s->graph_steps[s->cur] != nsteps) {
+        if (gx) {
+            HIPCHK(hipGraphExecDestroy(gx));
+            gx = nullptr;
+            s->graph_steps[s->cur] = 0;
         }
-        // record the chunk without running it; the buffer parity returns to cur after an
-        // even number of steps, so every replay starts from the same buffer
+        // record the chunk without running it
         const long long steps0 = s->steps;
         const int cur0 = s->cur;
         const bool af = s->auto_force;
@@ -575,15 +615,15 @@ int islbm_step_graph(islbm_solver* s, int32_t nsteps)
             return rc;
         }
         if (e != hipSuccess) return fail(ISLBM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-        const hipError_t ei = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+        const hipError_t ei = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ei != hipSuccess) {
-            s->graph = nullptr;
+            gx = nullptr;
             return fail(ISLBM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
         }
-        s->graph_steps = nsteps;
+        s->graph_steps[s->cur] = nsteps;
     }
-    HIPCHK(hipGraphLaunch(s->graph, s->stream));
+    HIPCHK(hipGraphLaunch(gx, s->stream));
     s->steps += nsteps;
     return ISLBM_OK;
 }
@@ -617,6 +657,10 @@ int islbm_upload_macros(islbm_solver* s, const double* rho, const double* u, con
             h.assign(in[a], in[a] + s->n);
             HIPCHK(hipMemcpy(s->mac[a], h.data(), s->n * 4, hipMemcpyHostToDevice));
         }
+    }
+    if (v) {
+        if (int rc = flux_refresh(s)) return rc;
+        HIPCHK(hipStreamSynchronize(s->stream));
     }
     return ISLBM_OK;
 }

@@ -226,12 +226,13 @@ std::string build_host_geometry(const islbm_params& p, HostGeom& g)
         for (int k = 0; k < nz; ++k) zg[k + 1] = tanh_map(total, ms, g.a, k, nz - 1) + H;
         zg[nz + 1] = p.lz;
         g.fluxw.resize((size_t)nz * nx);
+        g.flux_dx.resize(nx);
+        g.flux_dz.resize(nz);
+        for (int i = 0; i < nx; ++i) g.flux_dx[i] = (xat(i + 1) - xat(i - 1)) / 2.0;
         for (int k = 0; k < nz; ++k) {
             const double dzl = (zg[k + 2] - zg[k]) / 2.0;
-            for (int i = 0; i < nx; ++i) {
-                const double dxl = (xat(i + 1) - xat(i - 1)) / 2.0;
-                g.fluxw[(size_t)k * nx + i] = dxl * dzl;
-            }
+            g.flux_dz[k] = dzl;
+            for (int i = 0; i < nx; ++i) g.fluxw[(size_t)k * nx + i] = g.flux_dx[i] * dzl;
         }
     }
 

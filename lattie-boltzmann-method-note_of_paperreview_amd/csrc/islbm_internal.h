@@ -22,6 +22,8 @@ struct HostGeom {
     std::vector<double> bq;              // [NBFL][2][ny]
     std::vector<double> bw;              // [NBW][2][ny][7]
     std::vector<double> fluxw;           // [nz][nx]  dx_i * dz_k on the flux probe plane
+    std::vector<double> flux_dx;         // [nx]      dx_i of the probe plane
+    std::vector<double> flux_dz;         // [nz]      dz_k of the probe plane
     std::vector<double> les_delta;       // [nz][ny]  (dx dy dz_local)^(1/3)
     int flux_plane = 0;                  // unique j of the probe plane (y = LY - 3 dy)
 };
@@ -86,6 +88,14 @@ struct StepLaunch {
     double tau, dt, cs_smag;
     const float* scale;   // per-direction 16-bit scale (device), or null
     void* hbuf;           // two-pass FP64: horizontally filtered directions 3-18, [16][n]
+    // bulk-flux probe fused into the tiled kernel (null flux_part: probed by launch_flux_probe)
+    double* flux_part;    // [2][flux_nslot] per-step partial sums, by step parity
+    const double* flux_dx;  // [nx] probe weights dx_i
+    const double* flux_dz;  // [nz] probe weights dz_k
+    double* flux_acc;     // the flux accumulator
+    int flux_plane;       // unique j of the probe plane
+    int flux_par;         // parity of this step: writes flux_part[par], adds flux_part[1 - par]
+    int flux_nslot;
 };
 hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L);
 hipError_t launch_stream_copy(hipStream_t st, const void* src, void* dst, long long bytes);
@@ -98,6 +108,13 @@ hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, long lo
                                    int ny, int nz, double rho0, const double u0[3], double amp, uint64_t seed,
                                    const float* scale);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
+// Slots of one step's probe partials written by the tiled kernel (z chunks x x tiles).
+int tiled_flux_slots(const LaunchGeom& g, Store store);
+// Sum of the current probe-plane flux sum_k,i v dx_i dz_k into part[0] and zeros into
+// part[1 .. nslot-1]: the partials the tiled kernel's next step adds, after v was set
+// from outside the step (start, upload, restart).
+hipError_t launch_flux_refresh(hipStream_t st, const void* v, bool fp64, const double* dx, const double* dz,
+                               double* part, int nslot, int nx, int ny, int nz, int jplane);
 
 // Turbulence statistics (islbm_stats.hip): NSTAT FP64 accumulators [a][k][j][i], names in
 // kStatNames; zdw = 5-point wall-normal derivative weights [j][k][5] (zderiv_weights).

@@ -177,6 +177,40 @@ hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const dou
     return hipGetLastError();
 }
 
+// The probe-plane flux of the current v into slot 0 of the tiled kernel's partials (the
+// other slots zero), in a fixed order: each thread sums its levels, then a tree over threads.
+template <class M>
+__global__ __launch_bounds__(256) void k_flux_refresh(const M* __restrict__ v, const double* __restrict__ dx,
+                                                      const double* __restrict__ dz, double* __restrict__ part,
+                                                      int nslot, int nx, int ny, int nz, int jp)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long long t = threadIdx.x; t < (long long)nx * nz; t += blockDim.x) {
+        const int k = (int)(t / nx), i = (int)(t % nx);
+        s += ((double)v[((long long)k * ny + jp) * nx + i] * dx[i]) * dz[k];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < nslot; e += blockDim.x) part[e] = e == 0 ? red[0] : 0.0;
+}
+
+hipError_t launch_flux_refresh(hipStream_t st, const void* v, bool fp64, const double* dx, const double* dz,
+                               double* part, int nslot, int nx, int ny, int nz, int jplane)
+{
+    if (fp64)
+        hipLaunchKernelGGL(k_flux_refresh<double>, dim3(1), dim3(256), 0, st, (const double*)v, dx, dz, part, nslot,
+                           nx, ny, nz, jplane);
+    else
+        hipLaunchKernelGGL(k_flux_refresh<float>, dim3(1), dim3(256), 0, st, (const float*)v, dx, dz, part, nslot,
+                           nx, ny, nz, jplane);
+    return hipGetLastError();
+}
+
 __global__ void k_force_update(double* force, double* acc, double inv_area, int ndtfrc, double beta, double uref)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;

@@ -83,6 +83,12 @@ struct StepArgs {
     int les;
     float scale[NQ];
     float inv_scale[NQ];
+    // bulk-flux probe fused into the tiled kernel (StepLaunch::flux_*)
+    double* flux_part;
+    const double* flux_dx;
+    const double* flux_dz;
+    double* flux_acc;
+    int flux_plane, flux_par, flux_nslot;
 };
 
 template <Store ST>
@@ -114,6 +120,13 @@ inline StepArgs<ST> make_step_args(const StepLaunch& L)
         a.scale[q] = L.scale ? L.scale[q] : 1.0f;
         a.inv_scale[q] = L.scale ? 1.0f / L.scale[q] : 1.0f;
     }
+    a.flux_part = L.flux_part;
+    a.flux_dx = L.flux_dx;
+    a.flux_dz = L.flux_dz;
+    a.flux_acc = L.flux_acc;
+    a.flux_plane = L.flux_plane;
+    a.flux_par = L.flux_par;
+    a.flux_nslot = L.flux_nslot;
     return a;
 }
 

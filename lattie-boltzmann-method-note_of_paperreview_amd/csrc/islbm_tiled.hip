@@ -793,6 +793,23 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
         const auto rfn = rsrc_of(a.fn, NQ * nbq);
         // rho, u, v, w are consecutive arrays of one allocation (checked at launch)
         const auto rmac = rsrc_of(a.rho, 4 * fbytes);
+        // Bulk-flux probe (AccumulateUbulk, evolution.h:826-840) fused into the step.  The
+        // reference samples v of the previous step on the probe plane before each step; here
+        // the waves of the probe row sum v dx_i dz_k of the values they store, per lane over
+        // the chunk's levels and then over the wave, into this step's slot (chunk, x tile),
+        // and the next step adds the slots of this one to the accumulator in a fixed order.
+        // Lanes outside the grid (partial tiles) carry weight 0.  The global loads below
+        // precede every store of the wave.
+        const bool probe = a.flux_part != nullptr && j == a.flux_plane;
+        double pdx = 0.0, psum = 0.0;
+        if (probe) pdx = i < nx ? a.flux_dx[iw] : 0.0;
+        if (a.flux_part != nullptr && wk == 0 && ty == 0 && lane == 0) {
+            // the previous step's partials (written by the previous launch), in slot order
+            const double* prev = a.flux_part + (long long)(1 - a.flux_par) * a.flux_nslot;
+            double t = 0.0;
+            for (int e = 0; e < a.flux_nslot; ++e) t += prev[e];
+            a.flux_acc[0] += t;
+        }
         // walls, collision and stores of target level tk.  Lanes outside the grid (partial
         // tiles) hold the wrapped column or row, whose values they compute bit for bit like
         // its owner, so their stores duplicate the owner's: the stores need no lane mask,
@@ -813,6 +830,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             float rho, ux, uy, uz, tau_eff;
             collide_pairs<true>(F, rho_mod, (float)a.dt, force, delta, a.les, (float)a.tau, (float)a.cs, rho, ux, uy, uz,
                                 tau_eff);
+            if (probe) psum += ((double)uy * pdx) * ((const __attribute__((address_space(4))) double*)a.flux_dz)[tk];
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
 #pragma unroll
@@ -849,8 +867,15 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
             pend = ho;
             ++s;
         }
+        if (probe) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) psum += __shfl_xor(psum, off);
+            if (lane == 0) a.flux_part[(long long)a.flux_par * a.flux_nslot + chunk * tg.ntx + tile % tg.ntx] = psum;
+        }
     }
 }
+
+bool tiled_wide_staging(const LaunchGeom& g, Store store);
 
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
 {
@@ -859,6 +884,14 @@ bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
     const long long sz = store == Store::F32 ? 4 : 2;
     return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && NQ * g.qs * sz < (1LL << 32) &&
            4 * g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
+}
+
+int tiled_flux_slots(const LaunchGeom& g, Store store)
+{
+    const bool wide = tiled_wide_staging(g, store);
+    const TiledGeom t = wide ? tiled_geom(g.nx, g.ny, g.nz, Lay<true>::TY, Lay<true>::RESIDENT)
+                             : tiled_geom(g.nx, g.ny, g.nz, Lay<false>::TY, Lay<false>::RESIDENT);
+    return t.nchunks * t.ntx;
 }
 
 template <Store ST, bool W>

@@ -138,6 +138,14 @@ int islbm_get_params(islbm_solver* s, islbm_params* p);
  * FP64 device reduction.  The reference's computation of rho_modify lives in its absent
  * main; this restores the mean density to rho0 in one step. */
 int islbm_mass_correct(islbm_solver* s, double rho0, double* mean_rho);
+/* islbm_mass_correct without any device allocation by the library: no counterpart in the
+ * reference.  For callers that cannot allocate (graph-captured or tight serving loops,
+ * harnesses that forbid hipMalloc).  The reduction's partial sums live in one static
+ * device array of the library per GPU (513 doubles, held until the process ends); it fits
+ * every grid, since the partial count does not grow with the grid.  Results are identical
+ * to islbm_mass_correct.  Calls that overlap on one GPU (solvers on different streams or
+ * threads) share that array and race: serialise them. */
+int islbm_mass_correct_noalloc(islbm_solver* s, double rho0, double* mean_rho);
 /* Launch_Monitor (monitor.h:4-26): out = [v, u, w, force] at the reference's monitor node
  * (NX6/2, NYD6/2, NZ6/2 of the padded single-rank layout). */
 int islbm_monitor(islbm_solver* s, double out[4]);

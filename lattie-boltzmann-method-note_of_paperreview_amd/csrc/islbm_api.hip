@@ -680,6 +680,26 @@ int islbm_mass_correct(islbm_solver* s, double rho0, double* mean_rho)
     return ISLBM_OK;
 }
 
+int islbm_mass_correct_noalloc(islbm_solver* s, double rho0, double* mean_rho)
+{
+    if (!s) return fail(ISLBM_ERR_ARG, "null solver");
+    HIPCHK(hipSetDevice(s->p.device));
+    double* scr = nullptr;
+    HIPCHK(sum_scratch(s->p.device, &scr));
+    // the reduction fills all kSumBlocks partials and the sum whatever the grid size, so
+    // every solver fits the scratch and nothing in it has to start at zero
+    static_assert(kSumBlocks + 1 <= kSumScratch, "reduction scratch");
+    HIPCHK(launch_sum(s->stream, s->fp64, s->mac[0], s->n, scr, scr + kSumBlocks));
+    double sum = 0.0;
+    HIPCHK(hipMemcpyAsync(&sum, scr + kSumBlocks, sizeof sum, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const double mean = sum / (double)s->n;
+    const double dr = rho0 - mean;
+    HIPCHK(hipMemcpy(s->scal + 1, &dr, sizeof dr, hipMemcpyHostToDevice));
+    if (mean_rho) *mean_rho = mean;
+    return ISLBM_OK;
+}
+
 int islbm_monitor(islbm_solver* s, double out[4])
 {
     if (!s || !out) return fail(ISLBM_ERR_ARG, "null argument");

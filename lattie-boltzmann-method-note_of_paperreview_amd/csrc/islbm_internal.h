@@ -125,6 +125,11 @@ hipError_t launch_stats(hipStream_t st, bool fp64, const void* mac, double* acc,
 // deterministic sum of n values (compute precision) into out[0]; part holds kSumBlocks partials
 constexpr int kSumBlocks = 512;
 hipError_t launch_sum(hipStream_t st, bool fp64, const void* x, long long n, double* part, double* out);
+// Device address, on `device` (current on the calling thread), of the library's static
+// reduction scratch: kSumScratch doubles (kSumBlocks partials + the sum).
+constexpr int kSumScratch = kSumBlocks + 1;
+constexpr int kMaxDevices = 64;
+hipError_t sum_scratch(int device, double** out);
 
 hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* part,
                              double* acc, int nx, int ny, int nz, int jplane);

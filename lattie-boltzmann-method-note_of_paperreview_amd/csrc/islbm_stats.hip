@@ -9,6 +9,8 @@
 // uses (n = k - 2, clamped to the column, statistics.h:196-199).
 #include "islbm_internal.h"
 
+#include <mutex>
+
 namespace islbm {
 
 namespace {
@@ -132,6 +134,31 @@ hipError_t launch_stats(hipStream_t st, bool fp64, const void* mac, double* acc,
     else
         hipLaunchKernelGGL(k_stats<float>, dim3(nb), dim3(256), 0, st, (const float*)mac, acc, zdw, ix, iy, nx, ny, nz);
     return hipGetLastError();
+}
+
+// Scratch of the allocation-free reduction (islbm_mass_correct_noalloc): the kSumBlocks
+// partials and the sum, one copy per GPU for the life of the process.  Host code never
+// uses this name as an address: the device address comes from sum_scratch() below.
+static __device__ __align__(256) double scratch[kSumScratch];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t sum_scratch(int device, double** out)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // the calling thread has `device` current (the entry point sets it)
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = (double*)g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_sum(hipStream_t st, bool fp64, const void* x, long long n, double* part, double* out)

@@ -34,7 +34,7 @@ HEADER_SYMBOLS = [
     "islbm_batch_create", "islbm_batch_destroy", "islbm_batch_submit", "islbm_batch_advance",
     "islbm_batch_status", "islbm_batch_result", "islbm_batch_summary", "islbm_batch_bytes_in_use",
     "islbm_batch_last_error", "islbm_batch_now",
-    "islbm_upload_macros", "islbm_get_params", "islbm_mass_correct", "islbm_monitor",
+    "islbm_upload_macros", "islbm_get_params", "islbm_mass_correct", "islbm_mass_correct_noalloc", "islbm_monitor",
     "islbm_stats_enable", "islbm_stats_disable", "islbm_stats_accumulate", "islbm_stats_reset", "islbm_stats_count",
     "islbm_stats_set_count", "islbm_stats_names", "islbm_stats_download", "islbm_stats_upload",
     "islbm_checkpoint_write", "islbm_checkpoint_read", "islbm_stats_write", "islbm_stats_read",
@@ -125,6 +125,7 @@ def lib():
     L.islbm_upload_macros.argtypes = [vp, dp, dp, dp, dp]
     L.islbm_get_params.argtypes = [vp, pp]
     L.islbm_mass_correct.argtypes = [vp, ctypes.c_double, dp]
+    L.islbm_mass_correct_noalloc.argtypes = [vp, ctypes.c_double, dp]
     L.islbm_monitor.argtypes = [vp, dp]
     L.islbm_stats_enable.argtypes = [vp, i32]
     for nm in ("islbm_stats_disable", "islbm_stats_accumulate", "islbm_stats_reset"):

@@ -45,6 +45,16 @@ extern "C" {
                                 (GetNonuniParameter, initializationTool.h:16-39)         */
 #define ISLBM_MESH_STRETCH 1 /* a = tanh(stretch / 2)                                   */
 
+/* wall-normal (xi) interpolation next to the walls */
+#define ISLBM_XI_REFERENCE 0      /* the reference's 7-point Lagrange weights at every level,
+                                     extrapolating where a departure point lies between a
+                                     wall and the first or last node (initialization.h:185-258) */
+#define ISLBM_XI_CLAMPED_LINEAR 1 /* opt-in stabilisation, not in the reference: the departure
+                                     xi is clamped to [xi of the first node, xi of the last
+                                     node] and the four levels next to each wall (whose 7-point
+                                     stencil is one-sided) interpolate linearly between the
+                                     bracketing nodes; all other levels as the reference    */
+
 /* step kernel */
 #define ISLBM_KERNEL_AUTO 0      /* FP64: two-pass; else tiled where supported, else per-node */
 #define ISLBM_KERNEL_PERNODE 1   /* one thread per node, direct neighbour reads          */
@@ -72,6 +82,7 @@ typedef struct islbm_params {
     int32_t precision;     /* ISLBM_FP64 / FP32 / BF16 / FP16                             */
     int32_t kernel;        /* ISLBM_KERNEL_*                                              */
     int32_t device;        /* HIP device ordinal                                          */
+    int32_t xi_near_wall;  /* ISLBM_XI_REFERENCE (default) or ISLBM_XI_CLAMPED_LINEAR        */
 } islbm_params;
 
 typedef struct islbm_solver islbm_solver;

@@ -222,7 +222,7 @@ int upload_tables(islbm_solver* s)
     HIPCHK(hipMemcpy(dy, g.y.data(), g.y.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dxi, g.xi.data(), g.xi.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(launch_xi_weights(s->stream, d64, dzc, dy, dxi, g.nx, g.ny, g.nz, s->p.ly, s->p.lz, s->p.lxi,
-                             s->p.hill_alpha, s->p.flat, g.minsize));
+                             s->p.hill_alpha, s->p.flat, g.minsize, s->p.xi_near_wall));
     if (sizeof(C) == 8) {
         s->xiw = d64;
         d64 = nullptr;

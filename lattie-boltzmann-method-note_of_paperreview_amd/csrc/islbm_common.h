@@ -151,6 +151,31 @@ __host__ __device__ inline int xi_stencil_start(int k, int nz)
     return k <= 3 ? 0 : (k >= nz - 4 ? nz - 7 : k - 3);
 }
 
+// xi weights of level k for a departure at xi coordinate pxi, over the 7 nodes of k's
+// stencil (xi = all nz node coordinates).  near_wall = 0: the reference's 7-point Lagrange
+// weights.  near_wall = 1 (ISLBM_XI_CLAMPED_LINEAR): pxi clamped to [xi[0], xi[nz-1]], and
+// on the levels whose stencil is one-sided (k <= 3, k >= nz-4) linear weights on the two
+// stencil nodes that bracket pxi.
+__host__ __device__ inline int xi_stencil_start(int k, int nz);
+__host__ __device__ inline void xi_weights7(double pxi, const double* xi, int k, int nz, int near_wall, double* out7)
+{
+    const int s = xi_stencil_start(k, nz);
+    const double* nodes = xi + s;
+    if (near_wall) {
+        pxi = pxi < xi[0] ? xi[0] : (pxi > xi[nz - 1] ? xi[nz - 1] : pxi);
+        if (k <= 3 || k >= nz - 4) {
+            int m = 0;
+            while (m < 5 && nodes[m + 1] < pxi) ++m;
+            const double t = (pxi - nodes[m]) / (nodes[m + 1] - nodes[m]);
+            for (int r = 0; r < 7; ++r) out7[r] = 0.0;
+            out7[m] = 1.0 - t;
+            out7[m + 1] = t;
+            return;
+        }
+    }
+    for (int m = 0; m < 7; ++m) out7[m] = lagrange7_weight(pxi, nodes, m);
+}
+
 __host__ __device__ inline int wrap_index(int a, int n)
 {
     a %= n;

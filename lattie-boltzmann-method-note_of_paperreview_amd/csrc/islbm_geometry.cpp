@@ -94,6 +94,8 @@ std::string build_host_geometry(const islbm_params& p, HostGeom& g)
     if (p.cs_smag < 0) return "cs_smag must be >= 0";
     if (p.mesh_mode != ISLBM_MESH_BISECT && p.mesh_mode != ISLBM_MESH_STRETCH) return "unknown mesh_mode";
     if (p.mesh_mode == ISLBM_MESH_STRETCH && !(p.stretch > 0)) return "stretch must be positive";
+    if (p.xi_near_wall != ISLBM_XI_REFERENCE && p.xi_near_wall != ISLBM_XI_CLAMPED_LINEAR)
+        return "unknown xi_near_wall";
 
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     g.nx = nx; g.ny = ny; g.nz = nz;
@@ -156,7 +158,10 @@ std::string build_host_geometry(const islbm_params& p, HostGeom& g)
         const double H = hill_profile(py, ly, al, flat);
         const double L = p.lz - H - ms;
         const double pxi = p.lxi * (pz - (H + ms / 2.0)) / L;
-        lagrange7(out7, pxi, g.xi.data());
+        if (p.xi_near_wall)
+            xi_weights7(pxi, g.xi.data(), 0, nz, 1, out7);  // layers 0 and 1 share stencil 0
+        else
+            lagrange7(out7, pxi, g.xi.data());
     };
 
     // BFLInitialization: flags, q and weights on the first two layers

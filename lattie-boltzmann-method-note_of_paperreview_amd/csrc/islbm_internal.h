@@ -68,7 +68,7 @@ long long pop_stride(long long n, int bytes);
 // Kernel launchers (islbm_kernels.hip).
 hipError_t launch_xi_weights(hipStream_t st, double* out, const double* zc, const double* y,
                              const double* xi, int nx, int ny, int nz, double ly, double lz,
-                             double lxi, double alpha, int flat, double minsize);
+                             double lxi, double alpha, int flat, double minsize, int near_wall);
 hipError_t launch_convert_f64(hipStream_t st, const double* in, void* out, long long n, int to_fp32);
 
 struct StepLaunch {

@@ -25,7 +25,7 @@ namespace islbm {
 
 __global__ void k_xi_weights(double* __restrict__ out, const double* __restrict__ zc,
                              const double* __restrict__ y, const double* __restrict__ xi, int ny, int nz,
-                             double ly, double lz, double lxi, double alpha, int flat, double ms)
+                             double ly, double lz, double lxi, double alpha, int flat, double ms, int near_wall)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nz * ny * NXI) return;
@@ -38,19 +38,18 @@ __global__ void k_xi_weights(double* __restrict__ out, const double* __restrict_
     const double H = hill_profile(py, ly, alpha, flat);
     const double L = lz - H - ms;
     const double pxi = lxi * (pz - (H + ms / 2.0)) / L;
-    const double* nodes = xi + xi_stencil_start(k, nz);
     double* o = out + ((size_t)(k * ny + j) * NXI + f) * 7;
-    for (int m = 0; m < 7; ++m) o[m] = lagrange7_weight(pxi, nodes, m);
+    xi_weights7(pxi, xi, k, nz, near_wall, o);
 }
 
 hipError_t launch_xi_weights(hipStream_t st, double* out, const double* zc, const double* y, const double* xi,
                              int nx, int ny, int nz, double ly, double lz, double lxi, double alpha, int flat,
-                             double minsize)
+                             double minsize, int near_wall)
 {
     (void)nx;
     const int n = nz * ny * NXI;
     hipLaunchKernelGGL(k_xi_weights, dim3((n + 255) / 256), dim3(256), 0, st, out, zc, y, xi, ny, nz, ly, lz, lxi,
-                       alpha, flat, minsize);
+                       alpha, flat, minsize, near_wall);
     return hipGetLastError();
 }
 

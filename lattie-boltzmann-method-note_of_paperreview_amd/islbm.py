@@ -55,7 +55,7 @@ class Params(ctypes.Structure):
         ("flat", ctypes.c_int32), ("mesh_mode", ctypes.c_int32), ("stretch", ctypes.c_double),
         ("tau", ctypes.c_double), ("re", ctypes.c_double), ("u_ref", ctypes.c_double),
         ("cs_smag", ctypes.c_double), ("ndtfrc", ctypes.c_int32), ("precision", ctypes.c_int32),
-        ("kernel", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("kernel", ctypes.c_int32), ("device", ctypes.c_int32), ("xi_near_wall", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -162,7 +162,10 @@ def default_params(**kw) -> Params:
     """The reference's shipped parameter set (variables.h), updated by keyword overrides."""
     p = Params()
     _check(lib().islbm_default_params(ctypes.byref(p)), "islbm_default_params")
+    names = {f for f, _ in Params._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise KeyError(f"unknown parameter {k}")
         if k == "precision" and isinstance(v, str):
             v = PRECISIONS[v]
         setattr(p, k, v)

@@ -29,7 +29,7 @@ def workload(n: int, seed: int = 20261203, nsteps: int = 5000):
         re = float(np.exp(rng.uniform(np.log(100.0), np.log(5600.0))))
         alpha = float(rng.uniform(0.5, 3.0))
         kw = dict(nx=ny // 2, ny=ny, nz=nz, re=re, tau=0.0, u_ref=0.05, cs_smag=0.1, hill_alpha=alpha,
-                  ly=5.142 + 3.858 * alpha, mesh_mode=1, stretch=2.0)
+                  ly=5.142 + 3.858 * alpha, mesh_mode=1, stretch=2.0, xi_near_wall=1)
         out.append(dict(kw=kw, nsteps=nsteps, seed=seed + k))
     return out
 

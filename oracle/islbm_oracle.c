@@ -273,6 +273,23 @@ static void xi_weights(const or_state* s, double* out7, double pos_z, double pos
         n = s->NZ6 - 10;
     else
         n = K - 3;
+    if (p->xi_near_wall) {
+        /* opt-in near-wall treatment (no reference counterpart): the departure xi is clamped
+         * to [xi of the first node, xi of the last node]; the levels whose stencil is
+         * one-sided interpolate linearly between the two stencil nodes that bracket it */
+        const double lo = s->xi[3], hi = s->xi[s->NZ6 - 4];
+        if (pos_xi < lo) pos_xi = lo;
+        if (pos_xi > hi) pos_xi = hi;
+        if (K <= 6 || K >= s->NZ6 - 7) {
+            int m = 0;
+            while (m < 5 && s->xi[n + m + 1] < pos_xi) ++m;
+            const double t = (pos_xi - s->xi[n + m]) / (s->xi[n + m + 1] - s->xi[n + m]);
+            for (int r = 0; r < 7; ++r) out7[r] = 0.0;
+            out7[m] = 1.0 - t;
+            out7[m + 1] = t;
+            return;
+        }
+    }
     lagrange7(out7, pos_xi, s->xi, n);
 }
 

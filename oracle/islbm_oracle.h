@@ -38,6 +38,9 @@ typedef struct or_params {
     double u_ref;           /* used when tau <= 0: nu = u_ref/Re, tau = 3nu/dt + 1/2 */
     double cs_smag;         /* Smagorinsky constant (0 = reference MRT) */
     int    ndtfrc;          /* forcing-controller cadence (steps) */
+    int    xi_near_wall;    /* 0 = the reference's xi weights; 1 = departure xi clamped to the
+                               node range and linear weights on the 4 levels next to each wall
+                               (the library's opt-in ISLBM_XI_CLAMPED_LINEAR) */
 } or_params;
 
 typedef struct or_state or_state;

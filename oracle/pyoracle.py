@@ -26,7 +26,7 @@ class OrParams(ctypes.Structure):
         ("cfl", ctypes.c_double), ("lxi", ctypes.c_double), ("hill_alpha", ctypes.c_double),
         ("flat", ctypes.c_int), ("mesh_mode", ctypes.c_int), ("stretch", ctypes.c_double),
         ("tau", ctypes.c_double), ("re", ctypes.c_double), ("u_ref", ctypes.c_double),
-        ("cs_smag", ctypes.c_double), ("ndtfrc", ctypes.c_int),
+        ("cs_smag", ctypes.c_double), ("ndtfrc", ctypes.c_int), ("xi_near_wall", ctypes.c_int),
     ]
 
 
@@ -102,7 +102,7 @@ def default_params(**kw) -> OrParams:
     """Laminar periodic-hill workload, 32 x 64 x 64 (spanwise x streamwise x wall-normal)."""
     p = dict(nx=32, ny=64, nz=64, lx=4.5, ly=9.0, lz=3.036, cfl=0.6, lxi=10.0, hill_alpha=1.0,
              flat=0, mesh_mode=1, stretch=2.0, tau=0.0, re=200.0, u_ref=0.05, cs_smag=0.0,
-             ndtfrc=10000)
+             ndtfrc=10000, xi_near_wall=0)
     p.update(kw)
     return OrParams(**p)
 

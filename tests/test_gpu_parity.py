@@ -34,7 +34,9 @@ def compare(o, s, tol, what=""):
 
 
 @pytest.mark.parametrize("mode", [dict(), dict(mesh_mode=1, stretch=2.0), dict(flat=1),
-                                  dict(hill_alpha=1.7, ly=5.142 + 3.858 * 1.7)])
+                                  dict(hill_alpha=1.7, ly=5.142 + 3.858 * 1.7),
+                                  dict(mesh_mode=1, stretch=2.0, xi_near_wall=1),
+                                  dict(hill_alpha=0.8, ly=5.142 + 3.858 * 0.8, ny=48, xi_near_wall=1)])
 def test_geometry_tables_match_oracle(pkg, po, mode):
     o, s = pair(pkg, po, **mode)
     for name in ["x", "y", "xi", "z", "xw0", "xw2", "yw0", "yw2", "xiw3", "xiw4", "xiw5", "xiw6", "xiw15",
@@ -84,6 +86,34 @@ def test_fp64_parity_baseline_literal_early_steps(pkg, po):
     o.step(8)
     s.step(8)
     compare(o, s, TOL64, "baseline literal 8")
+
+
+# BASELINE config 1 as written (Re_h 200, U_b 0.05, tanh stretch 2.0), which the reference
+# scheme cannot run, with the opt-in near-wall xi treatment (ISLBM_XI_CLAMPED_LINEAR)
+CONFIG1 = dict(nx=32, ny=64, nz=64, tau=0.0, re=200.0, u_ref=0.05, mesh_mode=1, stretch=2.0, xi_near_wall=1)
+
+
+def test_fp64_parity_config1_clamped_xi_1000_steps(pkg, po):
+    o, s = pair(pkg, po, **CONFIG1)
+    assert s.kernel_in_use() == pkg.KERNEL_TWOPASS
+    for x in (o, s):
+        x.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=20261203)
+    for n in (10, 990):
+        o.step(n)
+        s.step(n)
+        compare(o, s, TOL64, f"config 1 clamped-xi +{n}")
+    r, u, v, w = s.macros()
+    assert np.all(np.isfinite(u)) and np.max(np.abs(v)) < 0.2
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
+def test_tiled_config1_clamped_xi_against_fp64_oracle(pkg, po, prec, tol):
+    o, s = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED, **CONFIG1)
+    for x in (o, s):
+        x.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=20261203)
+    o.step(200)
+    s.step(200)
+    compare(o, s, tol, f"tiled {prec} config 1 clamped-xi 200")
 
 
 def test_fp64_les_parity(pkg, po):

@@ -115,10 +115,11 @@ size_t state_bytes(const islbm_params& p)
            (uses_twopass(p) ? 16 * qs * sizeof(double) : 0);  // H rows at the population stride
 }
 
-__global__ void k_cast_f64(const double* in, float* out, long long n)
+// FP32 form of consecutive 7-point weight sets (round_weights7)
+__global__ void k_round_weights7(const double* in, float* out, long long ngroups)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) out[t] = (float)in[t];
+    if (t < ngroups) round_weights7(in + 7 * t, out + 7 * t);
 }
 
 }  // namespace
@@ -183,9 +184,12 @@ void free_all(islbm_solver* s)
 }
 
 template <typename C>
-int upload_table(islbm_solver* s, void** dst, const std::vector<double>& src)
+int upload_table(islbm_solver* s, void** dst, const std::vector<double>& src, bool weights7 = false)
 {
     std::vector<C> tmp(src.begin(), src.end());
+    if constexpr (sizeof(C) == 4)
+        if (weights7)  // consecutive 7-point weight sets: rounded to keep their sums
+            for (size_t g = 0; g + 7 <= src.size(); g += 7) round_weights7(&src[g], &tmp[g]);
     HIPCHK(hipMalloc(dst, tmp.size() * sizeof(C) + 8));
     HIPCHK(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(C), hipMemcpyHostToDevice));
     return ISLBM_OK;
@@ -196,9 +200,9 @@ int upload_tables(islbm_solver* s)
 {
     const HostGeom& g = s->g;
     int rc;
-    if ((rc = upload_table<C>(s, &s->xw0, g.xw0)) || (rc = upload_table<C>(s, &s->xw2, g.xw2)) ||
-        (rc = upload_table<C>(s, &s->yw0, g.yw0)) || (rc = upload_table<C>(s, &s->yw2, g.yw2)) ||
-        (rc = upload_table<C>(s, &s->bq, g.bq)) || (rc = upload_table<C>(s, &s->bw, g.bw)) ||
+    if ((rc = upload_table<C>(s, &s->xw0, g.xw0, true)) || (rc = upload_table<C>(s, &s->xw2, g.xw2, true)) ||
+        (rc = upload_table<C>(s, &s->yw0, g.yw0, true)) || (rc = upload_table<C>(s, &s->yw2, g.yw2, true)) ||
+        (rc = upload_table<C>(s, &s->bq, g.bq)) || (rc = upload_table<C>(s, &s->bw, g.bw, true)) ||
         (rc = upload_table<C>(s, &s->les, g.les_delta)))
         return rc;
     // BFL branch codes decided in FP64 (see apply_walls)
@@ -228,8 +232,9 @@ int upload_tables(islbm_solver* s)
         d64 = nullptr;
     } else {
         HIPCHK(hipMalloc(&s->xiw, nxi * sizeof(float)));
-        hipLaunchKernelGGL(k_cast_f64, dim3((unsigned)((nxi + 255) / 256)), dim3(256), 0, s->stream, d64,
-                           (float*)s->xiw, (long long)nxi);
+        const long long ng = (long long)(nxi / 7);
+        hipLaunchKernelGGL(k_round_weights7, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s->stream, d64,
+                           (float*)s->xiw, ng);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1199,7 +1204,8 @@ double collision_forms_gap(int les, uint64_t& state)
         }
         C r1, u1, v1, w1, t1, r2, u2, v2, w2, t2;
         collide_node<kDev>(A, (C)1e-5, (C)0.0190875, (C)2e-4, (C)0.05, les, (C)0.6, (C)0.1, r1, u1, v1, w1, t1);
-        collide_pairs<kDev>(B, (C)1e-5, (C)0.0190875, (C)2e-4, (C)0.05, les, (C)0.6, (C)0.1, r2, u2, v2, w2, t2);
+        collide_pairs<kDev>(B, (C)1e-5, (C)0.0190875, (C)2e-4, (C)0.05, les, (C)0.6, (C)0.1, r2, u2, v2, w2, t2,
+                            (C)(2 * 0.6 - 1));
         for (int q = 0; q < NQ; ++q) worst = std::max(worst, std::fabs((double)A[q] - (double)B[q]));
     }
     return worst;

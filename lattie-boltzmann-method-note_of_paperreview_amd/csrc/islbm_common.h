@@ -176,6 +176,59 @@ __host__ __device__ inline void xi_weights7(double pxi, const double* xi, int k,
     for (int m = 0; m < 7; ++m) out7[m] = lagrange7_weight(pxi, nodes, m);
 }
 
+// FP32 form of one 7-point weight set.  Reduced-precision solves interpolate the deviation
+// f - w_q, so the FP32 weights' errors in the moments sum_l w_l (l-3)^p act once per step:
+// a sum of 1 + e scales every interpolated deviation by 1 + e (a drift of N e after N
+// steps, about 1e-8 per step with rounding to nearest), a first-moment error shifts the
+// departure point and a second-moment error changes the interpolation's diffusion.  The
+// weights are rounded to nearest, then the FP64 residuals of the sum and of the first two
+// moments are moved onto three of the weights (the 3 x 3 Vandermonde solve on their
+// offsets, each corrected weight rounded again), keeping the choice of three with the
+// smallest remaining |d sum| + |d m1| + |d m2| (about 1e-10 instead of 1e-8); every
+// weight stays within a few FP32 ulps of the largest weight of its FP64 value.
+__host__ __device__ inline void round_weights7(const double* w, float* out)
+{
+    float r[7];
+    double R0 = 0.0, R1 = 0.0, R2 = 0.0;
+    for (int l = 0; l < 7; ++l) {
+        r[l] = (float)w[l];
+        const double e = w[l] - (double)r[l], x = (double)(l - 3);
+        R0 += e;
+        R1 += e * x;
+        R2 += e * x * x;
+    }
+    auto cost = [&](const float* v) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int l = 0; l < 7; ++l) {
+            const double e = w[l] - (double)v[l], x = (double)(l - 3);
+            s0 += e;
+            s1 += e * x;
+            s2 += e * x * x;
+        }
+        return fabs(s0) + fabs(s1) + fabs(s2);
+    };
+    for (int l = 0; l < 7; ++l) out[l] = r[l];
+    double best = cost(r);
+    for (int a = 0; a < 7; ++a)
+        for (int b = a + 1; b < 7; ++b)
+            for (int c = b + 1; c < 7; ++c) {
+                const double xa = a - 3, xb = b - 3, xc = c - 3;
+                const double da = (R2 - (xb + xc) * R1 + xb * xc * R0) / ((xa - xb) * (xa - xc));
+                const double db = (R2 - (xa + xc) * R1 + xa * xc * R0) / ((xb - xa) * (xb - xc));
+                const double dc = (R2 - (xa + xb) * R1 + xa * xb * R0) / ((xc - xa) * (xc - xb));
+                float v[7];
+                for (int l = 0; l < 7; ++l) v[l] = r[l];
+                v[a] = (float)((double)r[a] + da);
+                v[b] = (float)((double)r[b] + db);
+                v[c] = (float)((double)r[c] + dc);
+                const double cc = cost(v);
+                if (cc < best) {
+                    best = cc;
+                    for (int l = 0; l < 7; ++l) out[l] = v[l];
+                }
+            }
+}
+
 __host__ __device__ inline int wrap_index(int a, int n)
 {
     a %= n;

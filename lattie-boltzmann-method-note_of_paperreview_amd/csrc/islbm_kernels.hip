@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_collide_pernode(StepArgs<ST> a)
     C tau_eff = (C)a.tau;
     C rho, ux, uy, uz;
     collide_pairs<CD::kDev>(F, (C)*a.rho_modify, (C)a.dt, (C)(*a.force), a.les ? a.t.les_delta[(long long)k * ny + j] : C(0),
-                           a.les, (C)a.tau, (C)a.cs, rho, ux, uy, uz, tau_eff);
+                           a.les, (C)a.tau, (C)a.cs, rho, ux, uy, uz, tau_eff, (C)(2.0 * a.tau - 1.0));
 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) CD::st(a.fn + (long long)q * a.qs, node, F[q], a.scale[q]);
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(256) void k_collide_inplace(StepArgs<Store::F64> a)
     double tau_eff = a.tau;
     double rho, ux, uy, uz;
     collide_pairs<CD::kDev>(F, *a.rho_modify, a.dt, *a.force, a.les ? a.t.les_delta[(long long)k * a.ny + j] : 0.0, a.les,
-                            a.tau, a.cs, rho, ux, uy, uz, tau_eff);
+                            a.tau, a.cs, rho, ux, uy, uz, tau_eff, 2.0 * a.tau - 1.0);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) CD::st(a.fn + (long long)q * a.qs, node, F[q], a.scale[q]);
     a.rho[node] = rho;

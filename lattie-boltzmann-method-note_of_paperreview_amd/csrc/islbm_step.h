@@ -474,9 +474,10 @@ __host__ __device__ inline C sqrt_(C x)
     return sqrt(x);
 }
 
+// tau_m = 2 tau - 1, from FP64 (FP32 arithmetic only; see the stress rates below).
 template <bool kDev, class C>
 __host__ __device__ inline void collide_pairs(C (&F)[NQ], C rho_modify, C dt, C force, C delta, int les, C tau, C cs,
-                                              C& rho_out, C& ux_out, C& uy_out, C& uz_out, C& tau_eff)
+                                              C& rho_out, C& ux_out, C& uy_out, C& uz_out, C& tau_eff, C tau_m)
 {
     constexpr bool kFast = use_fast_math<C>();
     F[0] = F[0] + rho_modify;
@@ -529,6 +530,14 @@ __host__ __device__ inline void collide_pairs(C (&F)[NQ], C rho_modify, C dt, C 
     const C m18 = a11 + a12 - a15 - a16;
 
     C s_nu = rcp_(tau);
+    // FP32: the viscosity (tau - 1/2) dt / 3 = (1/s_nu - 1/2) dt / 3 depends on the
+    // distance eps = 2 - s_nu of the stress rate from 2, small as tau nears 1/2 (0.004 at
+    // tau 0.501).  An FP32 s_nu near 2 is off by up to ~1e-7, a viscosity error of
+    // ~1e-7 / eps; so eps itself is carried, from tau_m = 2 tau - 1 formed in FP64:
+    // eps = (2 tau - 1) / tau, or with the Smagorinsky increment dl,
+    // eps = (2 tau - 1 + 2 dl) / (tau + dl).
+    C eps = C(0);
+    if constexpr (kFast) eps = tau_m * rcp_(tau);
     tau_eff = tau;
     if (les) {
         const C pxx = s1 + exy + exz, pyy = s3 + exy + eyz, pzz = s5 + exz + eyz;
@@ -536,20 +545,40 @@ __host__ __device__ inline void collide_pairs(C (&F)[NQ], C rho_modify, C dt, C 
         const C Cd = cs * cs * delta * delta;
         const C b = tau * dt;
         if constexpr (kFast) {
-            const C S = (sqrt_(b * b + C(18.0 * 1.4142135623730951) * Cd * sqrt_(Q) * irho) - b) * rcp_(C(6) * Cd);
-            tau_eff = tau + C(3) * Cd * S * rcp_(dt);
+            // 3 Cd |S| / dt with |S| = (sqrt(b^2 + x) - b) / (6 Cd) in the form without the
+            // cancellation of sqrt(b^2 + x) - b (x / b^2 ~ 1e-7 in weak flow, where FP32 would
+            // leave |S| mostly rounding noise): x / (2 dt (sqrt(b^2 + x) + b))
+            const C x = C(18.0 * 1.4142135623730951) * Cd * sqrt_(Q) * irho;
+            const C dl = x * rcp_(C(2) * dt * (sqrt_(b * b + x) + b));
+            tau_eff = tau + dl;
+            eps = (tau_m + C(2) * dl) * rcp_(tau_eff);
         } else {
             const C S = (sqrt(b * b + C(18.0 * 1.4142135623730951) * Cd * sqrt(Q) * irho) - b) / (C(6) * Cd);
             tau_eff = tau + C(3) * Cd * S / dt;
         }
         s_nu = rcp_(tau_eff);
     }
+    // stress rate over the row norm n: s_nu / n, or in FP32 (2 - eps) / n as hi + (lo - eps / n)
+    // with hi + lo = 2 / n to FP64 accuracy
+    auto srate = [&](C m, auto nc) -> C {
+        constexpr double n = decltype(nc)::value;
+        if constexpr (kFast) {
+            constexpr C hi = C(2.0 / n);
+            constexpr C lo = C(2.0 / n - (double)hi);
+            return m * hi + m * (lo - eps * C(1.0 / n));
+        } else {
+            return m * (s_nu * C(1.0 / n));
+        }
+    };
+    using n36 = std::integral_constant<int, 36>;
+    using n12 = std::integral_constant<int, 12>;
+    using n4 = std::integral_constant<int, 4>;
     // r_a = s_a m_a / |M_a|^2
     const C r1 = m1 * C(1.19 / 2394.0), r2 = m2 * C(1.4 / 252.0);
     const C r4 = m4 * C(1.2 / 40.0), r6 = m6 * C(1.2 / 40.0), r8 = m8 * C(1.2 / 40.0);
-    const C r9 = m9 * (s_nu * C(1.0 / 36.0)), r10 = m10 * C(1.4 / 72.0);
-    const C r11 = m11 * (s_nu * C(1.0 / 12.0)), r12 = m12 * C(1.4 / 24.0);
-    const C r13 = m13 * (s_nu * C(0.25)), r14 = m14 * (s_nu * C(0.25)), r15 = m15 * (s_nu * C(0.25));
+    const C r9 = srate(m9, n36{}), r10 = m10 * C(1.4 / 72.0);
+    const C r11 = srate(m11, n12{}), r12 = m12 * C(1.4 / 24.0);
+    const C r13 = srate(m13, n4{}), r14 = srate(m14, n4{}), r15 = srate(m15, n4{});
     const C r16 = m16 * C(1.5 / 8.0), r17 = m17 * C(1.5 / 8.0), r18 = m18 * C(1.5 / 8.0);
 
     // back-transform: even part E per pair class, odd part O per pair

@@ -829,7 +829,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
                                       : 0.f;
             float rho, ux, uy, uz, tau_eff;
             collide_pairs<true>(F, rho_mod, (float)a.dt, force, delta, a.les, (float)a.tau, (float)a.cs, rho, ux, uy, uz,
-                                tau_eff);
+                                tau_eff, (float)(2.0 * a.tau - 1.0));
             if (probe) psum += ((double)uy * pdx) * ((const __attribute__((address_space(4))) double*)a.flux_dz)[tk];
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;

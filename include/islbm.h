@@ -250,6 +250,8 @@ int islbm_batch_result(islbm_batch* b, int64_t id, double* rho, double* u, doubl
 /* [nodes, mean rho, max |u|, mean v] of a finished request. */
 int islbm_batch_summary(islbm_batch* b, int64_t id, double out[4]);
 int islbm_batch_bytes_in_use(islbm_batch* b, size_t* bytes);
+/* out = [grouped chunks launched, times the grouped chunk graph was recorded]. */
+int islbm_batch_group_stats(islbm_batch* b, int64_t out[2]);
 const char* islbm_batch_last_error(void);
 double islbm_batch_now(void);
 

@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -327,15 +328,15 @@ int get_f(islbm_solver* s, int buf, double* f)
     return ISLBM_OK;
 }
 
-StepLaunch make_launch(islbm_solver* s)
+StepLaunch make_launch(islbm_solver* s, int cur)
 {
     StepLaunch L{};
     L.store = s->store;
     L.fp64 = s->fp64;
     L.kernel = s->kernel;
     L.g = LaunchGeom{s->p.nx, s->p.ny, s->p.nz, s->n, s->qs};
-    L.fo = s->f[s->cur];
-    L.fn = s->f[1 - s->cur];
+    L.fo = s->f[cur];
+    L.fn = s->f[1 - cur];
     L.rho = s->mac[0];
     L.u = s->mac[1];
     L.v = s->mac[2];
@@ -353,7 +354,7 @@ StepLaunch make_launch(islbm_solver* s)
     L.flux_dz = s->flux_dxz ? s->flux_dxz + s->p.nx : nullptr;
     L.flux_acc = s->scal + 2;
     L.flux_plane = s->g.flux_plane;
-    L.flux_par = s->cur;
+    L.flux_par = cur;
     L.flux_nslot = s->flux_nslot;
     return L;
 }
@@ -369,11 +370,11 @@ int flux_refresh(islbm_solver* s)
     return ISLBM_OK;
 }
 
-int enqueue_force_update(islbm_solver* s)
+int enqueue_force_update(islbm_solver* s, hipStream_t st = nullptr)
 {
     const double beta = (0.001 > 3.0 / s->p.re) ? 0.001 : 3.0 / s->p.re;
-    HIPCHK(launch_force_update(s->stream, s->scal + 0, s->scal + 2, 1.0 / (double)(s->p.lx * (s->p.lz - 1.0)),
-                               s->p.ndtfrc, beta, s->g.uref));
+    HIPCHK(launch_force_update(st ? st : s->stream, s->scal + 0, s->scal + 2,
+                               1.0 / (double)(s->p.lx * (s->p.lz - 1.0)), s->p.ndtfrc, beta, s->g.uref));
     return ISLBM_OK;
 }
 
@@ -384,7 +385,7 @@ int enqueue_step(islbm_solver* s, hipEvent_t ev0, hipEvent_t ev1)
         HIPCHK(launch_flux_probe(s->stream, s->mac[2], s->fp64, s->fluxw, s->fluxw + (size_t)s->p.nx * s->p.nz,
                                  s->scal + 2, s->p.nx, s->p.ny, s->p.nz, s->g.flux_plane));
     if (ev0) HIPCHK(hipEventRecord(ev0, s->stream));
-    StepLaunch L = make_launch(s);
+    StepLaunch L = make_launch(s, s->cur);
     HIPCHK(launch_stream_collide(s->stream, L));
     if (ev1) HIPCHK(hipEventRecord(ev1, s->stream));
     s->cur = 1 - s->cur;
@@ -1049,6 +1050,51 @@ int islbm_copy_bandwidth(int32_t device, size_t bytes, int32_t iters, double* gb
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Grouped stepping of several solves (the serving loop's batches)
+// ------------------------------------------------------------------------------------
+bool islbm::group_eligible(islbm_solver* s)
+{
+    const LaunchGeom lg{s->p.nx, s->p.ny, s->p.nz, s->n, s->qs};
+    return s->kernel == ISLBM_KERNEL_TILED && tiled_groupable(lg, s->store) && !(s->st_acc && s->st_every > 0) &&
+           s->auto_force;
+}
+
+int islbm::group_limit(islbm_solver* s, int64_t want)
+{
+    const int64_t to_force = s->p.ndtfrc - s->steps % s->p.ndtfrc;  // steps up to the next controller update
+    return (int)std::min<int64_t>(want, to_force);
+}
+
+int islbm::group_parity(islbm_solver* s) { return s->cur; }
+
+// Enqueues max(limit) grouped launches on st (member e takes limit[e] steps) and, for a
+// member whose chunk ends on a controller step, its forcing update; host state untouched.
+int islbm::group_record(hipStream_t st, islbm_solver* const* ss, const int* limit, int n)
+{
+    if (n <= 0 || n > kGroupMaxSolves) return fail(ISLBM_ERR_ARG, "group of %d solves", n);
+    int steps = 0;
+    for (int e = 0; e < n; ++e) steps = std::max(steps, limit[e]);
+    StepLaunch L[kGroupMaxSolves];
+    for (int t = 0; t < steps; ++t) {
+        for (int e = 0; e < n; ++e) L[e] = make_launch(ss[e], ss[e]->cur ^ (t & 1));
+        HIPCHK(launch_tiled_group(st, L, limit, n, t));
+    }
+    for (int e = 0; e < n; ++e)
+        if ((ss[e]->steps + limit[e]) % ss[e]->p.ndtfrc == 0)
+            if (int rc = enqueue_force_update(ss[e], st)) return rc;
+    return ISLBM_OK;
+}
+
+// The host state after a recorded chunk: buffer parity and step count of each member.
+void islbm::group_commit(islbm_solver* const* ss, const int* limit, int n)
+{
+    for (int e = 0; e < n; ++e) {
+        ss[e]->cur ^= (limit[e] & 1);
+        ss[e]->steps += limit[e];
+    }
+}
 
 // ------------------------------------------------------------------------------------
 // Reference-named entry points on the reference's padded FP64 layout

@@ -108,6 +108,24 @@ hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, long lo
                                    int ny, int nz, double rho0, const double u0[3], double amp, uint64_t seed,
                                    const float* scale);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
+// Grouped stepping (serving): one launch advances up to 8 FP32 solves whose grids take the
+// tiled kernel with 16-byte staging (tiled_groupable) by one step each; solve e takes part
+// while step < limit[e].  Results equal those of launch_stream_collide for each solve.
+bool tiled_groupable(const LaunchGeom& g, Store store);
+hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* limit, int n, int step);
+constexpr int kGroupMaxSolves = 8;
+// Host side of grouped stepping (islbm_api.hip), for the serving loop:
+//   group_eligible: the solve can join a group (FP32, 16-byte-staged tiled kernel, no
+//                   per-step statistics, automatic forcing updates);
+//   group_limit:    steps it may take in a chunk of `want` without passing a controller update;
+//   group_parity:   its current population buffer;
+//   group_record:   enqueue one chunk on st (capturable: host state untouched);
+//   group_commit:   advance the host state by the chunk.
+bool group_eligible(islbm_solver* s);
+int group_limit(islbm_solver* s, int64_t want);
+int group_parity(islbm_solver* s);
+int group_record(hipStream_t st, islbm_solver* const* ss, const int* limit, int n);
+void group_commit(islbm_solver* const* ss, const int* limit, int n);
 // Slots of one step's probe partials written by the tiled kernel (z chunks x x tiles).
 int tiled_flux_slots(const LaunchGeom& g, Store store);
 // Sum of the current probe-plane flux sum_k,i v dx_i dz_k into part[0] and zeros into

@@ -5,14 +5,20 @@
 // Launch_CollisionStreaming over one global state).  Here a request is one periodic-hill
 // solve with its own grid, Reynolds number, hill steepness and step count.  Requests are
 // admitted in submission order while the device-memory budget and the slot count allow;
-// an admitted solve keeps its populations resident, runs on its own stream and advances in
-// chunks replayed from a HIP graph (islbm_step_graph), two chunks in flight, so the solves
-// overlap on the device and a request that arrives while others run starts as soon as a
-// slot and memory free up.  A finished solve is retired (fields downloaded if requested,
-// a summary kept) and its memory returned to the budget.
-#include "../../include/islbm.h"
-
+// an admitted solve keeps its populations resident.  Up to 8 admitted solves that take the
+// FP32 tiled kernel advance together: each step of all of them is one grouped launch
+// (launch_tiled_group), and chunks of those launches are replayed from a HIP graph
+// recorded for the current members, two chunks in flight on the batch's stream; the graph
+// is recorded again only when the members, their step limits, their buffer parities or
+// their forcing updates change.  Other solves (FP64, 16-bit storage, grids the grouped
+// kernel does not take) run on their own streams in graph-replayed chunks
+// (islbm_step_graph).  A request that arrives while others run starts as soon as a slot
+// and memory free up.  A finished solve is retired (fields downloaded if requested, a
+// summary kept) and its memory returned to the budget.
 #include <hip/hip_runtime.h>
+
+#include "../../include/islbm.h"
+#include "islbm_internal.h"
 
 #include <algorithm>
 #include <chrono>
@@ -65,6 +71,7 @@ struct Request {
     hipEvent_t ev[kDepth] = {nullptr, nullptr};
     int64_t ev_steps[kDepth] = {0, 0};  // steps covered by the chunk behind each event
     int head = 0, inflight = 0;         // ring of in-flight chunk events
+    bool grouped = false;               // advanced by the batch's grouped launches
     std::vector<double> fields;         // rho u v w after completion (keep != 0)
     double summary[4] = {0, 0, 0, 0};   // nodes, mean rho, max |u|, mean v
     std::string error;
@@ -82,6 +89,17 @@ struct islbm_batch {
     std::vector<Request*> active;
     std::map<int64_t, Request*> all;
     int64_t n_done = 0, n_failed = 0;
+    // grouped stepping: the batch's stream, a ring of in-flight chunks (members and their
+    // steps per chunk), and the recorded chunk graph with the key it was recorded for
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev[kDepth] = {nullptr, nullptr};
+    std::vector<Request*> gmem[kDepth];
+    std::vector<int> glim[kDepth];
+    int ghead = 0, ginflight = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<int64_t> gkey;
+    int64_t n_group_records = 0;  // times the chunk graph was recorded
+    int64_t n_group_chunks = 0;   // grouped chunks launched
 };
 
 namespace {
@@ -102,6 +120,9 @@ void release(islbm_batch* b, Request* r)
 
 void set_failed(islbm_batch* b, Request* r, const char* why)
 {
+    // a grouped solve that fails may still be read by chunks in flight on the batch's
+    // stream (a finished one is not: no chunk after its last takes it)
+    if (r->grouped && r->s && b->gstream) (void)hipStreamSynchronize(b->gstream);
     r->error = why ? why : "";
     r->state = ISLBM_REQ_FAILED;
     r->inflight = 0;
@@ -155,6 +176,67 @@ int feed(islbm_batch* b, Request* r)
     return ISLBM_OK;
 }
 
+// The next chunk of the grouped solves: members (in admission order), their step limits,
+// recorded into a graph when the key changed, replayed on the batch's stream.
+int feed_group(islbm_batch* b)
+{
+    while (b->ginflight < kDepth) {
+        std::vector<Request*> mem;
+        std::vector<islbm_solver*> ss;
+        std::vector<int> lim;
+        std::vector<int64_t> key;
+        for (Request* r : b->active) {
+            if (!r->grouped || r->state != ISLBM_REQ_ACTIVE || r->queued_steps >= r->nsteps) continue;
+            const int l = islbm::group_limit(r->s, std::min<int64_t>(b->chunk, r->nsteps - r->queued_steps));
+            mem.push_back(r);
+            ss.push_back(r->s);
+            lim.push_back(l);
+            // request ids are never reused (a new solve may take a freed solve's address)
+            key.insert(key.end(), {r->id, l, islbm::group_parity(r->s)});
+        }
+        if (mem.empty()) return ISLBM_OK;
+        // a chunk that ends on a member's forcing update records that update too
+        for (size_t e = 0; e < mem.size(); ++e) {
+            int64_t st = 0;
+            islbm_step_count(mem[e]->s, &st);
+            key.push_back((st + lim[e]) % mem[e]->p.ndtfrc == 0);
+        }
+        if (!b->gexec || key != b->gkey) {
+            if (b->gexec) {
+                (void)hipGraphExecDestroy(b->gexec);
+                b->gexec = nullptr;
+            }
+            if (hipStreamBeginCapture(b->gstream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return bfail(ISLBM_ERR_HIP, "group capture");
+            const int rc = islbm::group_record(b->gstream, ss.data(), lim.data(), (int)ss.size());
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->gstream, &g);
+            if (rc || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : bfail(ISLBM_ERR_HIP, "group capture: %s", hipGetErrorString(e));
+            }
+            const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                b->gexec = nullptr;
+                return bfail(ISLBM_ERR_HIP, "group graph: %s", hipGetErrorString(ei));
+            }
+            b->gkey = key;
+            b->n_group_records++;
+        }
+        if (hipGraphLaunch(b->gexec, b->gstream) != hipSuccess) return bfail(ISLBM_ERR_HIP, "group graph launch");
+        b->n_group_chunks++;
+        const int slot = (b->ghead + b->ginflight) % kDepth;
+        if (hipEventRecord(b->gev[slot], b->gstream) != hipSuccess) return bfail(ISLBM_ERR_HIP, "event record");
+        islbm::group_commit(ss.data(), lim.data(), (int)ss.size());
+        b->gmem[slot] = mem;
+        b->glim[slot] = lim;
+        for (size_t e = 0; e < mem.size(); ++e) mem[e]->queued_steps += lim[e];
+        b->ginflight++;
+    }
+    return ISLBM_OK;
+}
+
 int admit(islbm_batch* b, Request* r)
 {
     islbm_params p = r->p;
@@ -165,6 +247,9 @@ int admit(islbm_batch* b, Request* r)
     for (hipEvent_t& e : r->ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bfail(ISLBM_ERR_HIP, "event create");
     if ((rc = islbm_init_equilibrium(r->s, 1.0, r->u0, r->amp, r->seed))) return rc;
+    int grouped = 0;
+    for (Request* a : b->active) grouped += a->grouped;
+    r->grouped = grouped < islbm::kGroupMaxSolves && islbm::group_eligible(r->s);
     r->state = ISLBM_REQ_ACTIVE;
     r->t_admit = now_s();
     return ISLBM_OK;
@@ -191,6 +276,12 @@ int islbm_batch_create(int32_t device, size_t hbm_budget, int32_t max_active, in
     b->budget = hbm_budget;
     b->max_active = max_active;
     b->chunk = chunk;
+    bool ok = hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking) == hipSuccess;
+    for (hipEvent_t& e : b->gev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        islbm_batch_destroy(b);
+        return bfail(ISLBM_ERR_HIP, "batch stream or events");
+    }
     *out = b;
     return ISLBM_OK;
 }
@@ -199,10 +290,15 @@ int islbm_batch_destroy(islbm_batch* b)
 {
     if (!b) return ISLBM_OK;
     (void)hipSetDevice(b->device);
+    if (b->gstream) (void)hipStreamSynchronize(b->gstream);
     for (auto& kv : b->all) {
         release(b, kv.second);
         delete kv.second;
     }
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    for (hipEvent_t e : b->gev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->gstream) (void)hipStreamDestroy(b->gstream);
     delete b;
     return ISLBM_OK;
 }
@@ -241,8 +337,24 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
     if (!b) return bfail(ISLBM_ERR_ARG, "null batch");
     if (hipSetDevice(b->device) != hipSuccess) return bfail(ISLBM_ERR_HIP, "hipSetDevice");
     for (int pass = 0; pass < 2; ++pass) {
-        // retire completed chunks, in order per solve
+        // retire completed grouped chunks, in order
         bool progressed = false;
+        while (b->ginflight > 0) {
+            const hipError_t q = hipEventQuery(b->gev[b->ghead]);
+            if (q == hipErrorNotReady) break;
+            const std::vector<Request*>& mem = b->gmem[b->ghead];
+            for (size_t e = 0; e < mem.size(); ++e) {
+                if (q != hipSuccess)
+                    set_failed(b, mem[e], hipGetErrorString(q));
+                else
+                    mem[e]->done_steps += b->glim[b->ghead][e];
+            }
+            b->gmem[b->ghead].clear();
+            b->ghead = (b->ghead + 1) % kDepth;
+            b->ginflight--;
+            progressed = true;
+        }
+        // retire completed chunks, in order per solve
         for (size_t t = 0; t < b->active.size();) {
             Request* r = b->active[t];
             while (r->state == ISLBM_REQ_ACTIVE && r->inflight > 0) {
@@ -283,16 +395,27 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
         }
         // keep every active solve fed
         for (Request* r : b->active) {
+            if (r->grouped) continue;
             int rc = feed(b, r);
             if (rc) set_failed(b, r, islbm_last_error());
         }
+        if (int rc = feed_group(b)) {
+            const std::string why = g_berr.empty() ? islbm_last_error() : g_berr;
+            for (Request* r : b->active)
+                if (r->grouped && r->state == ISLBM_REQ_ACTIVE) set_failed(b, r, why.c_str());
+            (void)rc;
+        }
         if (!wait || progressed || b->active.empty()) break;
-        // block until the oldest in-flight chunk of the first active solve completes
-        for (Request* r : b->active)
-            if (r->inflight > 0) {
-                (void)hipEventSynchronize(r->ev[r->head]);
-                break;
-            }
+        // block until the oldest in-flight chunk (grouped first) completes
+        if (b->ginflight > 0) {
+            (void)hipEventSynchronize(b->gev[b->ghead]);
+        } else {
+            for (Request* r : b->active)
+                if (r->inflight > 0) {
+                    (void)hipEventSynchronize(r->ev[r->head]);
+                    break;
+                }
+        }
     }
     if (out) {
         out[0] = (int64_t)b->queue.size();
@@ -348,6 +471,14 @@ int islbm_batch_bytes_in_use(islbm_batch* b, size_t* bytes)
 {
     if (!b || !bytes) return bfail(ISLBM_ERR_ARG, "null argument");
     *bytes = b->used;
+    return ISLBM_OK;
+}
+
+int islbm_batch_group_stats(islbm_batch* b, int64_t out[2])
+{
+    if (!b || !out) return bfail(ISLBM_ERR_ARG, "null argument");
+    out[0] = b->n_group_chunks;
+    out[1] = b->n_group_records;
     return ISLBM_OK;
 }
 

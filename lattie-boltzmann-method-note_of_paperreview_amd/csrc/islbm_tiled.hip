@@ -284,9 +284,11 @@ __device__ inline float dec16(unsigned short h)
         return (float)__builtin_bit_cast(_Float16, h);
 }
 
+// One workgroup's work of the stream-collide step of one solve (bid: its block index
+// within that solve's launch).  Inlined into the single-solve kernel and the grouped one.
 template <Store ST, bool W>
-__global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled(StepArgs<ST> a,
-                                                                                                          TiledGeom tg)
+__device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<ST>& a, const TiledGeom& tg,
+                                                                  const int bid)
 {
     using CD = Codec<ST>;
     using raw = typename CD::raw;
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
 
     // XCD-aware work mapping: block b runs on XCD b % 8
     const int per_xcd = tg.nblocks / 8;
-    const int wk = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int wk = (bid % 8) * per_xcd + bid / 8;
     if (wk >= tg.nwork) return;
     const int chunk = wk / tg.ntiles;
     const int tile = wk % tg.ntiles;
@@ -875,6 +877,41 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     }
 }
 
+template <Store ST, bool W>
+__global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled(StepArgs<ST> a,
+                                                                                                          TiledGeom tg)
+{
+    tiled_body<ST, W>(a, tg, (int)blockIdx.x);
+}
+
+// Several solves advanced by one launch (one step of each): solve e owns blocks
+// [block0[e], block0[e + 1]), each range starting at a multiple of 8 so that a block runs on
+// the XCD its solve's work mapping expects; a solve whose step limit within the current
+// chunk is reached takes no part.  The per-solve arguments stay in the kernel-argument
+// segment, read with scalar loads like those of the single-solve kernel.
+constexpr int kGroupMax = 8;
+template <Store ST>
+struct GroupArgs {
+    StepArgs<ST> a[kGroupMax];
+    TiledGeom tg[kGroupMax];
+    int block0[kGroupMax + 1];
+    int limit[kGroupMax];  // steps of the chunk the solve takes
+    int n;                 // solves
+    int step;              // index of this launch within the chunk
+};
+static_assert(sizeof(GroupArgs<Store::F32>) <= 4096, "grouped kernel arguments exceed 4 KiB");
+
+template <Store ST, bool W>
+__global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled_group(
+    GroupArgs<ST> g)
+{
+    const int b = (int)blockIdx.x;
+    int e = 0;
+    while (e + 1 < g.n && b >= g.block0[e + 1]) ++e;
+    if (g.step >= g.limit[e] || b - g.block0[e] >= g.tg[e].nblocks) return;
+    tiled_body<ST, W>(g.a[e], g.tg[e], b - g.block0[e]);
+}
+
 bool tiled_wide_staging(const LaunchGeom& g, Store store);
 
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
@@ -910,6 +947,36 @@ bool tiled_wide_staging(const LaunchGeom& g, Store store)
     return store == Store::F32 ? g.nx % 4 == 0 : g.nx % 8 == 0;
 }
 
+bool tiled_groupable(const LaunchGeom& g, Store store)
+{
+    return store == Store::F32 && tiled_supported(g, store, false) && tiled_wide_staging(g, store);
+}
+
+hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* limit, int n, int step)
+{
+    if (n <= 0 || n > kGroupMax) return hipErrorInvalidValue;
+    GroupArgs<Store::F32> g{};
+    int blocks = 0;
+    for (int e = 0; e < n; ++e) {
+        if (L[e].store != Store::F32 || !tiled_wide_staging(L[e].g, L[e].store)) return hipErrorInvalidValue;
+        const size_t fb = (size_t)L[e].g.n * 4;
+        const char* r = (const char*)L[e].rho;
+        if ((const char*)L[e].u != r + fb || (const char*)L[e].v != r + 2 * fb || (const char*)L[e].w != r + 3 * fb)
+            return hipErrorInvalidValue;
+        g.a[e] = make_step_args<Store::F32>(L[e]);
+        g.tg[e] = tiled_geom(L[e].g.nx, L[e].g.ny, L[e].g.nz, Lay<true>::TY, Lay<true>::RESIDENT);
+        g.block0[e] = blocks;
+        g.limit[e] = limit[e];
+        blocks += g.tg[e].nblocks;  // a multiple of 8
+    }
+    g.block0[n] = blocks;
+    g.n = n;
+    g.step = step;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled_group<Store::F32, true>), dim3(blocks), dim3(Lay<true>::NTH), 0,
+                       st, g);
+    return hipGetLastError();
+}
+
 hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
 {
     // the kernel addresses rho, u, v, w through one buffer resource
@@ -918,8 +985,15 @@ hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
     if ((const char*)L.u != r + fb || (const char*)L.v != r + 2 * fb || (const char*)L.w != r + 3 * fb)
         return hipErrorInvalidValue;
     switch (L.store) {
-    case Store::F32:
-        return tiled_wide_staging(L.g, L.store) ? launch_t<Store::F32, true>(st, L) : launch_t<Store::F32, false>(st, L);
+    case Store::F32: {
+        // a solve that a group can take runs the grouped kernel alone, so that its steps are
+        // the same compiled code whether it runs alone or in a group
+        if (tiled_groupable(L.g, L.store)) {
+            const int one = 1;
+            return launch_tiled_group(st, &L, &one, 1, 0);
+        }
+        return launch_t<Store::F32, false>(st, L);
+    }
     case Store::BF16:
         return tiled_wide_staging(L.g, L.store) ? launch_t<Store::BF16, true>(st, L) : launch_t<Store::BF16, false>(st, L);
     case Store::FP16:

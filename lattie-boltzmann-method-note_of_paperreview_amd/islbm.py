@@ -33,7 +33,7 @@ HEADER_SYMBOLS = [
     "islbm_step_graph", "islbm_stream", "islbm_device_bytes",
     "islbm_batch_create", "islbm_batch_destroy", "islbm_batch_submit", "islbm_batch_advance",
     "islbm_batch_status", "islbm_batch_result", "islbm_batch_summary", "islbm_batch_bytes_in_use",
-    "islbm_batch_last_error", "islbm_batch_now",
+    "islbm_batch_last_error", "islbm_batch_now", "islbm_batch_group_stats",
     "islbm_upload_macros", "islbm_get_params", "islbm_mass_correct", "islbm_mass_correct_noalloc", "islbm_monitor",
     "islbm_stats_enable", "islbm_stats_disable", "islbm_stats_accumulate", "islbm_stats_reset", "islbm_stats_count",
     "islbm_stats_set_count", "islbm_stats_names", "islbm_stats_download", "islbm_stats_upload",
@@ -120,6 +120,7 @@ def lib():
     L.islbm_batch_result.argtypes = [vp, i64, dp, dp, dp, dp]
     L.islbm_batch_summary.argtypes = [vp, i64, dp]
     L.islbm_batch_bytes_in_use.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.islbm_batch_group_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.islbm_batch_last_error.restype = ctypes.c_char_p
     L.islbm_batch_now.restype = ctypes.c_double
     L.islbm_upload_macros.argtypes = [vp, dp, dp, dp, dp]
@@ -451,6 +452,12 @@ class Batch:
         v = ctypes.c_size_t()
         _bcheck(lib().islbm_batch_bytes_in_use(self.h, ctypes.byref(v)), "islbm_batch_bytes_in_use")
         return v.value
+
+    def group_stats(self) -> dict:
+        """Grouped chunks launched and times their graph was recorded."""
+        o = (ctypes.c_int64 * 2)()
+        _bcheck(lib().islbm_batch_group_stats(self.h, o), "islbm_batch_group_stats")
+        return {"chunks": int(o[0]), "records": int(o[1])}
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20, device: int = 0) -> float:

@@ -109,6 +109,43 @@ def test_results_are_the_same_alone_and_in_the_mix(pkg):
             print(f"request {rid}: identical alone and in the mix, {st['steps']} steps", flush=True)
 
 
+def test_grouped_launches_equal_solo_runs(pkg):
+    """Eight FP32 solves of different grids advance together (one launch per step for all
+    of them, chunks replayed from a recorded graph); odd step counts end chunks early and
+    flip buffer parities, and forcing updates every 7 steps fall inside chunks.  Each
+    result equals the same solve run alone, bit for bit."""
+    grids = [(32, 64, 48), (48, 96, 64), (64, 128, 32), (32, 64, 96), (96, 64, 40), (16, 32, 32), (64, 48, 56),
+             (128, 96, 24)]
+    steps = [33, 41, 27, 50, 19, 36, 45, 23]
+    # the near-wall xi treatment the serving workload uses (the reference's weights diverge
+    # on some of these grids within 40 steps)
+    reqs = [(params(pkg, nx=g[0], ny=g[1], nz=g[2], precision="fp32", cs_smag=0.1 * (k % 2), ndtfrc=7,
+                    xi_near_wall=1), n, 50 + k)
+            for k, (g, n) in enumerate(zip(grids, steps))]
+    alone = [run_alone(pkg, p, n, sd) for p, n, sd in reqs]
+    with pkg.Batch(max_active=8, chunk=10) as b:
+        ids = [b.submit(p, n, u0=(0.0, 0.05, 0.0), amp=1e-3, seed=sd, keep_fields=True) for p, n, sd in reqs]
+        done, failed = b.run()
+        assert (done, failed) == (8, 0)
+        gs = b.group_stats()
+        # chunks end on the forcing updates, so their limits change from chunk to chunk
+        print(f"grouped chunks {gs['chunks']}, graph recordings {gs['records']}", flush=True)
+        assert gs["chunks"] >= 5
+        for rid, ref in zip(ids, alone):
+            got = b.result(rid)
+            for x, y in zip(got, ref):
+                assert np.isfinite(x).all() and np.array_equal(x, y), rid
+    # without forcing updates the chunk graph is recorded once per change of members
+    with pkg.Batch(max_active=8, chunk=10) as b:
+        for k, g in enumerate(grids[:4]):
+            b.submit(params(pkg, nx=g[0], ny=g[1], nz=g[2], precision="fp32"), 60, u0=(0.0, 0.05, 0.0), amp=1e-3,
+                     seed=k)
+        assert b.run() == (4, 0)
+        gs = b.group_stats()
+        print(f"no forcing: grouped chunks {gs['chunks']}, graph recordings {gs['records']}", flush=True)
+        assert gs["chunks"] == 6 and gs["records"] == 1
+
+
 def test_admission_respects_budget_and_slots(pkg):
     p = params(pkg, nx=32, ny=64, nz=64, precision="fp32")
     one = pkg.required_bytes(p)

@@ -1,0 +1,96 @@
+"""BASELINE configurations 2 and 3 at their stated operating points, with the opt-in
+near-wall xi treatment (ISLBM_XI_CLAMPED_LINEAR) under which the reference scheme runs them
+(its own xi weights diverge there within a few hundred steps).
+
+Config 2: 128 x 256 x 128, Re_h 2800, U_b 0.05, Smagorinsky Cs 0.1, tanh stretch 2.0; FP64
+(two-pass kernel) and FP32 (tiled kernel) from the same seeded field u = (0, U_b, 0) + a
+1e-3 perturbation, 10,000 steps.  Errors of FP32 against FP64: the instantaneous velocity,
+and the 5,000-step time-averaged, spanwise-averaged streamwise velocity and Reynolds stress
+<u'v'> profiles (streamwise x wall-normal) at x/h = 0.5, 2, 4, 6, 8 (statistics.h:122-249).
+
+Config 3: 192 x 384 x 192, Re_h 10595, same start, 20,000 steps; BF16 and FP16 storage
+against FP64: the time-averaged streamwise velocity over the last 10,000 steps (sampled
+every 10 steps) and the separation and reattachment positions, where the spanwise- and
+time-averaged streamwise velocity of the first node layer changes sign past the crest.
+Bounds are about twice the measured values (recorded in the test output).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C2 = dict(nx=128, ny=256, nz=128, tau=0.0, re=2800.0, u_ref=0.05, cs_smag=0.1, mesh_mode=1, stretch=2.0,
+          xi_near_wall=1)
+C3 = dict(nx=192, ny=384, nz=192, tau=0.0, re=10595.0, u_ref=0.05, cs_smag=0.1, mesh_mode=1, stretch=2.0,
+          xi_near_wall=1)
+
+
+def run(pkg, kw, prec, nsteps, avg_from, every):
+    s = pkg.Solver(pkg.default_params(**kw, precision=prec))
+    s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
+    s.step(avg_from)
+    s.stats_enable(every)
+    s.step(nsteps - avg_from)
+    m = s.macros()
+    n = s.stats_count()
+    st = {a: s.stats(a) / n for a in ("V", "W", "VW")}
+    y = s.table("y")
+    s.close()
+    assert all(np.isfinite(a).all() for a in m), prec
+    return m, st, y
+
+
+def rel_velocity(a, b):
+    return float(np.sqrt(sum(np.sum((x - y) ** 2) for x, y in zip(a[1:], b[1:])) / sum(np.sum(y ** 2) for y in b[1:])))
+
+
+def sign_changes(v0, y):
+    """Separation (+ to -) and reattachment (- to +) of the first-layer streamwise velocity."""
+    sep = rea = None
+    for j in range(len(y) - 1):
+        a, b = v0[j], v0[j + 1]
+        if sep is None and a > 0 >= b:
+            sep = y[j] + (y[j + 1] - y[j]) * a / (a - b)
+        elif sep is not None and rea is None and a < 0 <= b:
+            rea = y[j] + (y[j + 1] - y[j]) * a / (a - b)
+    return sep, rea
+
+
+def test_config2_fp32_against_fp64_10000_steps(pkg):
+    m64, s64, y = run(pkg, C2, "fp64", 10000, 5000, 1)
+    m32, s32, _ = run(pkg, C2, "fp32", 10000, 5000, 1)
+    e = rel_velocity(m32, m64)
+    print(f"config 2, 10000 steps: FP32 instantaneous rel L2 u {e:.3e}", flush=True)
+    assert e <= 1e-5
+    dy = 9.0 / 256
+    for xh in (0.5, 2, 4, 6, 8):
+        j = int(round(xh / dy))
+        prof = {}
+        for p, st in (("fp64", s64), ("fp32", s32)):
+            prof[p] = (st["V"][:, j, :].mean(1), (st["VW"] - st["V"] * st["W"])[:, j, :].mean(1))
+        eU = float(np.linalg.norm(prof["fp32"][0] - prof["fp64"][0]) / np.linalg.norm(prof["fp64"][0]))
+        eR = float(np.linalg.norm(prof["fp32"][1] - prof["fp64"][1]) / np.linalg.norm(prof["fp64"][1]))
+        print(f"  x/h = {xh}: mean U rel L2 {eU:.2e}, <u'v'> rel L2 {eR:.2e} "
+              f"(max |<u'v'>| {np.abs(prof['fp64'][1]).max():.2e})", flush=True)
+        assert eU <= 5e-6 and eR <= 1e-5, (xh, eU, eR)
+
+
+@pytest.fixture(scope="module")
+def c3_fp64(pkg):
+    return run(pkg, C3, "fp64", 20000, 10000, 10)
+
+
+@pytest.mark.parametrize("prec,tol_mean,tol_pos", [("fp16", 0.02, 0.05), ("bf16", 0.6, 0.3)])
+def test_config3_16bit_mean_velocity_and_separation(pkg, c3_fp64, prec, tol_mean, tol_pos):
+    m64, s64, y = c3_fp64
+    m, st, _ = run(pkg, C3, prec, 20000, 10000, 10)
+    Vbar64, Vbar = s64["V"].mean(2), st["V"].mean(2)
+    e = float(np.linalg.norm(Vbar - Vbar64) / np.linalg.norm(Vbar64))
+    sr64 = sign_changes(s64["V"][0].mean(1), y)
+    sr = sign_changes(st["V"][0].mean(1), y)
+    print(f"config 3, {prec}, 20000 steps: time-averaged streamwise velocity rel L2 {e:.3e}; separation / "
+          f"reattachment x/h {sr[0]:.3f} / {sr[1]:.3f} (FP64 {sr64[0]:.3f} / {sr64[1]:.3f}); instantaneous "
+          f"rel L2 u {rel_velocity(m, m64):.3e}", flush=True)
+    assert None not in sr64 and None not in sr
+    assert e <= tol_mean
+    assert abs(sr[0] - sr64[0]) <= tol_pos and abs(sr[1] - sr64[1]) <= tol_pos

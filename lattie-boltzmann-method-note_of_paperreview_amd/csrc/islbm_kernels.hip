@@ -450,162 +450,6 @@ using d2 = HIP_vector_type<double, 2>;
 __device__ inline d2 ld2(const double* p) { return *reinterpret_cast<const d2*>(p); }
 __device__ inline void st2(double* p, double a, double b) { *reinterpret_cast<d2*>(p) = d2(a, b); }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_hfilter2(StepArgs<Store::F64> a,
-                                                                                          double* __restrict__ H)
-{
-    const int nx = a.nx, ny = a.ny;
-    int i, j, k;
-    long long gid;
-    if (!twopass_node<false, 2>(nx, ny, a.nz, i, j, k, gid)) return;
-    const long long plane = (long long)nx * ny;
-    int xp[5], yo[7];  // pair offsets of the x window; row offsets of the y taps
-#pragma unroll
-    for (int m = 0; m < 5; ++m) xp[m] = 2 * wrap_index(i / 2 - 2 + m, nx / 2);
-#pragma unroll
-    for (int l = 0; l < 7; ++l) yo[l] = wrap_index(j - 3 + l, ny) * nx;
-    const long long lev = (long long)k * plane;
-    using cdouble = const __attribute__((address_space(4))) double;
-    const cdouble* X0 = (cdouble*)a.t.xw0;
-    const cdouble* X2 = (cdouble*)a.t.xw2;
-    const cdouble* Y0 = (cdouble*)a.t.yw0;
-    const cdouble* Y2 = (cdouble*)a.t.yw2;
-    // x filter of one row for both nodes
-    auto xrow = [&](const double* pr, const cdouble* wx, double& r0, double& r1) {
-        double w[10];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            const d2 v = ld2(pr + xp[m]);
-            w[2 * m] = v.x;
-            w[2 * m + 1] = v.y;
-        }
-        r0 = w[1] * wx[0];
-        r1 = w[2] * wx[0];
-#pragma unroll
-        for (int l = 1; l < 7; ++l) {
-            r0 = r0 + w[1 + l] * wx[l];
-            r1 = r1 + w[2 + l] * wx[l];
-        }
-    };
-#pragma unroll
-    for (int q = 3; q < NQ; ++q) {
-        const double* p = a.fo + (long long)q * a.qs + lev;
-        const cdouble* wx = cx(q) > 0 ? X0 : X2;
-        const cdouble* wy = cy(q) > 0 ? Y0 : Y2;
-        double r0, r1;
-        if (cx(q) == 0 && cy(q) == 0) {  // 5, 6: at the nodes
-            const d2 v = ld2(p + (long long)j * nx + i);
-            r0 = v.x;
-            r1 = v.y;
-        } else if (cy(q) == 0) {  // 11-14: x only
-            xrow(p + (long long)j * nx, wx, r0, r1);
-        } else if (cx(q) == 0) {  // 3, 4, 15-18: y only
-            const double* pc = p + i;
-            d2 v = ld2(pc + yo[0]);
-            r0 = v.x * wy[0];
-            r1 = v.y * wy[0];
-#pragma unroll
-            for (int n = 1; n < 7; ++n) {
-                v = ld2(pc + yo[n]);
-                r0 = r0 + v.x * wy[n];
-                r1 = r1 + v.y * wy[n];
-            }
-        } else {  // 7-10: x inner, y outer
-            r0 = r1 = 0.0;
-#pragma unroll
-            for (int n = 0; n < 7; ++n) {
-                double i0, i1;
-                xrow(p + yo[n], wx, i0, i1);
-                r0 = (n == 0) ? i0 * wy[0] : r0 + i0 * wy[n];
-                r1 = (n == 0) ? i1 * wy[0] : r1 + i1 * wy[n];
-                asm volatile("" ::: "memory");  // one row of taps in flight at a time
-            }
-        }
-        st2(H + (long long)(q - 3) * a.qs + gid, r0, r1);
-        asm volatile("" ::: "memory");
-    }
-}
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xi_stream2(StepArgs<Store::F64> a,
-                                                                                            const double* __restrict__ H)
-{
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    int i, j, k;
-    long long gid;
-    if (!twopass_node<true, 2>(nx, ny, nz, i, j, k, gid)) return;
-    const long long plane = (long long)nx * ny;
-    int xp[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) xp[m] = 2 * wrap_index(i / 2 - 2 + m, nx / 2);
-    const long long row = (long long)k * plane + (long long)j * nx;
-    const long long node = row + i;
-    const long long col = (long long)xi_stencil_start(k, nz) * plane + (long long)j * nx + i;  // stencil base
-    const double* XI = a.t.xiw + ((long long)k * ny + j) * (NXI * 7);
-    double F0[NQ], F1[NQ];
-    auto xi = [&](int q, int fam) {
-        const double* h = H + (long long)(q - 3) * a.qs + col;
-        const double* wz = XI + fam * 7;
-        d2 v = ld2(h);
-        double r0 = v.x * wz[0], r1 = v.y * wz[0];
-#pragma unroll
-        for (int m = 1; m < 7; ++m) {
-            v = ld2(h + m * plane);
-            r0 = r0 + v.x * wz[m];
-            r1 = r1 + v.y * wz[m];
-        }
-        F0[q] = r0;
-        F1[q] = r1;
-    };
-    using cdouble = const __attribute__((address_space(4))) double;
-    auto ipx = [&](int q, const cdouble* wx) {
-        const double* pr = a.fo + (long long)q * a.qs + row;
-        double w[10];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            const d2 v = ld2(pr + xp[m]);
-            w[2 * m] = v.x;
-            w[2 * m + 1] = v.y;
-        }
-        double r0 = w[1] * wx[0], r1 = w[2] * wx[0];
-#pragma unroll
-        for (int l = 1; l < 7; ++l) {
-            r0 = r0 + w[1 + l] * wx[l];
-            r1 = r1 + w[2 + l] * wx[l];
-        }
-        F0[q] = r0;
-        F1[q] = r1;
-    };
-    // groups of two directions, so that the loads in flight stay within the registers
-    ipx(1, (cdouble*)a.t.xw0);
-    ipx(2, (cdouble*)a.t.xw2);
-    asm volatile("" ::: "memory");
-    xi(3, XI3);
-    xi(4, XI4);
-    asm volatile("" ::: "memory");
-    xi(5, XI5);
-    xi(6, XI6);
-    asm volatile("" ::: "memory");
-    xi(7, XI3);
-    xi(8, XI3);
-    asm volatile("" ::: "memory");
-    xi(9, XI4);
-    xi(10, XI4);
-    asm volatile("" ::: "memory");
-    xi(11, XI5);
-    xi(12, XI5);
-    asm volatile("" ::: "memory");
-    xi(13, XI6);
-    xi(14, XI6);
-    asm volatile("" ::: "memory");
-    xi(15, XI15);
-    xi(16, XI16);
-    asm volatile("" ::: "memory");
-    xi(17, XI17);
-    xi(18, XI18);
-    asm volatile("" ::: "memory");
-#pragma unroll
-    for (int q = 1; q < NQ; ++q) st2(a.fn + (long long)q * a.qs + node, F0[q], F1[q]);
-}
-
 // LDS form of the xi pass (even nx): a workgroup takes one chunk of 128 nodes (64 pairs) of
 // the (i, j) plane at 8 consecutive levels, one wave per level.  For each direction the
 // levels the 8 stencils share (at most 14) are staged once into LDS, so each value of H is

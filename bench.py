@@ -145,6 +145,52 @@ def serving(pkg, nreq: int, nsteps: int, ncmp: int):
     return out
 
 
+def dropin_timing(pkg, calls: int):
+    """Launch_CollisionStreaming (the reference-named entry point, evolution.h:842) on the
+    reference's padded FP64 layout at the bench shape: 19 + 19 caller-owned device arrays
+    of (NX+7)(NY+7)(NZ+6) doubles, starting from the rest state f = w_q; each call gathers the
+    padded arrays, steps, scatters back and synchronises, as the reference's step does."""
+    import ctypes as ct
+    L = pkg.lib()
+    hip = ct.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+    hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    hip.hipFree.argtypes = [ct.c_void_p]
+    nx, ny, nz = GRID["nx"], GRID["ny"], GRID["nz"]
+    npad = (nx + 7) * (ny + 7) * (nz + 6)
+    p = pkg.default_params(**GRID, tau=1.0, re=50.0, u_ref=0.0, mesh_mode=0, precision="fp64")
+    assert L.islbm_ref_init(ct.byref(p)) == 0, L.islbm_last_error()
+    w = [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12
+    bufs = []
+    try:
+        for q in range(38):
+            ptr = ct.c_void_p()
+            assert hip.hipMalloc(ct.byref(ptr), npad * 8) == 0
+            bufs.append(ptr)
+            h = np.full(npad, w[q % 19])
+            assert hip.hipMemcpy(ptr, h.ctypes.data, npad * 8, 1) == 0
+        mac = [ct.c_void_p() for _ in range(6)]
+        assert L.islbm_ref_device_arrays(*[ct.byref(x) for x in mac]) == 0
+        one = np.ones(npad)
+        assert hip.hipMemcpy(mac[3], one.ctypes.data, npad * 8, 1) == 0  # rho_d = 1
+        A = (ct.c_void_p * 19)(*[b.value for b in bufs[:19]])
+        B = (ct.c_void_p * 19)(*[b.value for b in bufs[19:]])
+        for t in range(4):  # warm-up
+            L.Launch_CollisionStreaming(*((A, B) if t % 2 == 0 else (B, A)))
+        t0 = time.perf_counter()
+        for t in range(calls):
+            L.Launch_CollisionStreaming(*((A, B) if t % 2 == 0 else (B, A)))
+        dt = (time.perf_counter() - t0) / calls
+        err = L.islbm_last_error().decode() if hasattr(L.islbm_last_error(), "decode") else ""
+    finally:
+        for b in bufs:
+            hip.hipFree(b)
+        L.islbm_ref_shutdown()
+    n = nx * ny * nz
+    return {"ms_per_call": round(dt * 1e3, 3), "mlups": round(n / dt / 1e6, 1), "calls": calls,
+            "layout": f"padded {nx + 7}x{ny + 7}x{nz + 6} FP64, 19 + 19 caller arrays", "last_error": err}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +207,7 @@ def main() -> int:
     ap.add_argument("--serving-steps", type=int, default=5000)
     ap.add_argument("--serving-compare", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--dropin", type=int, default=0, help="also time N Launch_CollisionStreaming calls (FP64)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures the one-GPU path only; this build has no multi-GPU decomposition "
@@ -231,6 +278,8 @@ def main() -> int:
         out["serving"] = serving(pkg, args.serving_requests, args.serving_steps, args.serving_compare)
     if not args.no_accuracy and args.precision != "fp64":
         out["accuracy_vs_fp64"] = accuracy_vs_fp64(pkg, args.precision, phys, args.accuracy_steps)
+    if args.dropin > 0:
+        out["dropin"] = dropin_timing(pkg, args.dropin)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(phys, args.cpu_steps)
     print(json.dumps(out))

@@ -92,12 +92,12 @@ size_t table_bytes(const islbm_params& p)
 
 long long islbm::pop_stride(long long n, int bytes)
 {
-    constexpr long long MiB = 1LL << 20;
+    constexpr long long H = 1LL << 19;  // 512 KiB
     const long long b = n * bytes;
-    if (b < 4 * MiB) return n;
-    long long m = (b + MiB - 1) / MiB;
+    if (b < 8 * H) return n;
+    long long m = (b + H - 1) / H;
     if (m % 2 == 0) ++m;
-    return m * MiB / bytes;
+    return m * H / bytes;
 }
 
 namespace {

@@ -61,8 +61,9 @@ struct LaunchGeom {
     long long qs;  // population stride: nodes from the array of direction q to that of q + 1 (>= n)
 };
 // Population stride of a grid of n nodes stored in `bytes`-byte values: n, or for arrays of
-// 4 MiB and more, n rounded up to an odd number of MiB (strides that are even multiples of
-// large powers of two, or not multiples of 1 MiB, run the stream-collide step measurably slower).
+// 4 MiB and more, n rounded up to an odd multiple of 512 KiB (strides that are even
+// multiples of large powers of two, or just short of them, run the stream-collide step
+// measurably slower).
 long long pop_stride(long long n, int bytes);
 
 // Kernel launchers (islbm_kernels.hip).

@@ -232,26 +232,34 @@ __device__ inline C ip_x_y_xi(const R& f, long long base, const int* xo, const i
 }
 
 // The two BFL gathers of the wall treatment, in the same summation order as ip_y_xi and
-// ip_x_y_xi but with every loop but the innermost rolled: they run on the first two
-// levels only, and fully unrolled their 49 addresses per xi level would set the register
-// allocation of the whole kernel they are inlined into.
+// ip_x_y_xi.  They run on the first two levels only, so the xi loop stays rolled: fully
+// unrolled, their 49 addresses per xi level would set the register allocation of the whole
+// kernel they are inlined into.  Within a xi level the loads are issued together before
+// the sums that use them (one memory round trip per level instead of one per tap), and the
+// weights come through the constant address space (scalar loads where the row is uniform).
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+
 template <class R, class C>
-__device__ inline C wall_y_xi(const R& f, long long base, const int* yo, long long plane, const C* wy, const C* wz)
+__device__ inline C wall_y_xi(const R& f, long long base, const int* yo, long long plane, cptr<C> wy, cptr<C> wz)
 {
     C r = C(0);
 #pragma unroll 1
     for (int m = 0; m < 7; ++m) {
-        C in = C(0);
-#pragma unroll 1
-        for (int n = 0; n < 7; ++n) in = (n == 0) ? f(base + m * plane + yo[0]) * wy[0] : in + f(base + m * plane + yo[n]) * wy[n];
+        C v[7];
+#pragma unroll
+        for (int n = 0; n < 7; ++n) v[n] = f(base + m * plane + yo[n]);
+        C in = v[0] * wy[0];
+#pragma unroll
+        for (int n = 1; n < 7; ++n) in = in + v[n] * wy[n];
         r = (m == 0) ? in * wz[0] : r + in * wz[m];
     }
     return r;
 }
 
 template <class R, class C>
-__device__ inline C wall_x_y_xi(const R& f, long long base, const int* xo, const int* yo, long long plane, const C* wx,
-                                const C* wy, const C* wz)
+__device__ inline C wall_x_y_xi(const R& f, long long base, const int* xo, const int* yo, long long plane, cptr<C> wx,
+                                cptr<C> wy, cptr<C> wz)
 {
     C r = C(0);
 #pragma unroll 1
@@ -260,9 +268,12 @@ __device__ inline C wall_x_y_xi(const R& f, long long base, const int* xo, const
 #pragma unroll 1
         for (int n = 0; n < 7; ++n) {
             const long long b = base + m * plane + yo[n];
-            C in = f(b + xo[0]) * wx[0];
+            C v[7];
 #pragma unroll
-            for (int l = 1; l < 7; ++l) in = in + f(b + xo[l]) * wx[l];
+            for (int l = 0; l < 7; ++l) v[l] = f(b + xo[l]);
+            C in = v[0] * wx[0];
+#pragma unroll
+            for (int l = 1; l < 7; ++l) in = in + v[l] * wx[l];
             mid = (n == 0) ? in * wy[0] : mid + in * wy[n];
         }
         r = (m == 0) ? mid * wz[0] : r + mid * wz[m];
@@ -276,57 +287,77 @@ __device__ inline C wall_x_y_xi(const R& f, long long base, const int* xo, const
 // the hill (evolution.h:505-572; weights read at layer*ny + j).  bflag holds the branch
 // chosen in FP64 on the host: 1 = linear (q > 1/2), 2 = interpolated (q < 1/2),
 // 3 = flagged with q == 1/2 (no change), 0 = not flagged.
+// In FP32 and 16-bit storage the stored values of the node that any rule of its level
+// reads (directions 3-18) are loaded in one batch before the rules run, so a wall level
+// costs one memory round trip for them rather than one per branch; the branch tables are
+// read through the constant address space.
 // ------------------------------------------------------------------------------------
 template <Store ST, class C>
 __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long node, int i, int j, int k,
                                    const int* xo, const int* yo, long long plane)
 {
-    const long long N = a.n;
+    if (k > 1 && k != a.nz - 1) return;
     auto at = [&](int q) { return Codec<ST>::ld(a.fo + (long long)q * a.qs, node, a.inv_scale[q]); };
+    // FP64 kernels read them where a rule uses them: held together they would raise the
+    // register allocation of those kernels
+    constexpr bool kBatch = !std::is_same_v<C, double>;
+    C o_[NQ];
+    if constexpr (kBatch) {
+#pragma unroll
+        for (int q = 3; q < NQ; ++q) o_[q] = at(q);
+    }
+    auto o = [&](int q) -> C {
+        if constexpr (kBatch)
+            return o_[q];
+        else
+            return at(q);
+    };
     if (k == 0) {
-        F[5] = at(6); F[11] = at(14); F[12] = at(13); F[15] = at(18); F[16] = at(17);
+        F[5] = o(6); F[11] = o(14); F[12] = o(13); F[15] = o(18); F[16] = o(17);
     }
     if (k == a.nz - 1) {
-        F[6] = at(5); F[13] = at(12); F[14] = at(11); F[17] = at(16); F[18] = at(15);
+        F[6] = o(5); F[13] = o(12); F[14] = o(11); F[17] = o(16); F[18] = o(15);
     }
     if (k <= 1) {
         const int nb = 2 * a.ny;
         const int ib = k * a.ny + j;
+        const cptr<int> bflag = (cptr<int>)a.t.bflag;
+        const cptr<C> bq = (cptr<C>)a.t.bq;
         auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
-        auto W = [&](int which) { return a.t.bw + ((long long)which * nb + ib) * 7; };
-        const int f3 = a.t.bflag[B3 * nb + ib], f4 = a.t.bflag[B4 * nb + ib];
-        const int f15 = a.t.bflag[B15 * nb + ib], f16 = a.t.bflag[B16 * nb + ib];
+        auto W = [&](int which) { return (cptr<C>)a.t.bw + ((long long)which * nb + ib) * 7; };
+        const int f3 = bflag[B3 * nb + ib], f4 = bflag[B4 * nb + ib];
+        const int f15 = bflag[B15 * nb + ib], f16 = bflag[B16 * nb + ib];
         if (f3 == 1) {
-            const C q = a.t.bq[B3 * nb + ib];
+            const C q = bq[B3 * nb + ib];
             const C c1 = C(1) / (C(2) * q), c2 = (C(2) * q - C(1)) / (C(2) * q);
-            F[3] = c1 * at(4) + c2 * at(3);
-            F[7] = c1 * at(10) + c2 * at(7);
-            F[8] = c1 * at(9) + c2 * at(8);
+            F[3] = c1 * o(4) + c2 * o(3);
+            F[7] = c1 * o(10) + c2 * o(7);
+            F[8] = c1 * o(9) + c2 * o(8);
         } else if (f3 == 2) {
             F[3] = wall_y_xi(pop(4), (long long)i, yo, plane, W(BY378), W(BXI378));
             F[7] = wall_x_y_xi(pop(10), 0LL, xo, yo, plane, W(BX37), W(BY378), W(BXI378));
             F[8] = wall_x_y_xi(pop(9), 0LL, xo, yo, plane, W(BX38), W(BY378), W(BXI378));
         }
         if (f4 == 1) {
-            const C q = a.t.bq[B4 * nb + ib];
+            const C q = bq[B4 * nb + ib];
             const C c1 = C(1) / (C(2) * q), c2 = (C(2) * q - C(1)) / (C(2) * q);
-            F[4] = c1 * at(3) + c2 * at(4);
-            F[9] = c1 * at(8) + c2 * at(9);
-            F[10] = c1 * at(7) + c2 * at(10);
+            F[4] = c1 * o(3) + c2 * o(4);
+            F[9] = c1 * o(8) + c2 * o(9);
+            F[10] = c1 * o(7) + c2 * o(10);
         } else if (f4 == 2) {
             F[4] = wall_y_xi(pop(3), (long long)i, yo, plane, W(BY4910), W(BXI4910));
             F[9] = wall_x_y_xi(pop(8), 0LL, xo, yo, plane, W(BX49), W(BY4910), W(BXI4910));
             F[10] = wall_x_y_xi(pop(7), 0LL, xo, yo, plane, W(BX410), W(BY4910), W(BXI4910));
         }
         if (f15 == 1) {
-            const C q = a.t.bq[B15 * nb + ib];
-            F[15] = (C(1) / (C(2) * q)) * at(18) + ((C(2) * q - C(1)) / (C(2) * q)) * at(15);
+            const C q = bq[B15 * nb + ib];
+            F[15] = (C(1) / (C(2) * q)) * o(18) + ((C(2) * q - C(1)) / (C(2) * q)) * o(15);
         } else if (f15 == 2) {
             F[15] = wall_y_xi(pop(18), (long long)i, yo, plane, W(BY15), W(BXI15));
         }
         if (f16 == 1) {
-            const C q = a.t.bq[B16 * nb + ib];
-            F[16] = (C(1) / (C(2) * q)) * at(17) + ((C(2) * q - C(1)) / (C(2) * q)) * at(16);
+            const C q = bq[B16 * nb + ib];
+            F[16] = (C(1) / (C(2) * q)) * o(17) + ((C(2) * q - C(1)) / (C(2) * q)) * o(16);
         } else if (f16 == 2) {
             F[16] = wall_y_xi(pop(17), (long long)i, yo, plane, W(BY16), W(BXI16));
         }

@@ -41,8 +41,8 @@ def test_required_bytes(pkg):
     p = pkg.default_params(nx=128, ny=256, nz=128, precision="fp32")
     n = 128 * 256 * 128
     b = pkg.required_bytes(p)
-    # population arrays of 4 MiB and more sit an odd number of MiB apart: 16 MiB -> 17 MiB
-    qb = 17 * 2 ** 20
+    # population arrays of 4 MiB and more sit an odd multiple of 512 KiB apart: 16 -> 16.5 MiB
+    qb = 33 * 2 ** 19
     assert 2 * 19 * qb + 4 * n * 4 <= b < 2 * 19 * qb + 4 * n * 4 + 32 * 2 ** 20
     p16 = pkg.default_params(nx=128, ny=256, nz=128, precision="bf16")
     assert pkg.required_bytes(p16) < b
@@ -50,8 +50,8 @@ def test_required_bytes(pkg):
 
 @pytest.mark.parametrize("grid,q32,q16", [
     ((32, 64, 64), 512 * 2 ** 10, 256 * 2 ** 10),  # under 4 MiB: no padding
-    ((128, 256, 128), 17 * 2 ** 20, 9 * 2 ** 20),  # 16 -> 17 MiB, 8 -> 9 MiB
-    ((128, 256, 120), 15 * 2 ** 20, 9 * 2 ** 20),  # 15 MiB stays, 7.5 -> 8 -> 9 MiB
+    ((128, 256, 128), 33 * 2 ** 19, 17 * 2 ** 19),  # 16 -> 16.5 MiB, 8 -> 8.5 MiB
+    ((128, 256, 120), 31 * 2 ** 19, 15 * 2 ** 19),  # 15 -> 15.5 MiB, 7.5 MiB stays
 ])
 def test_population_stride(pkg, grid, q32, q16):
     # FP32 and BF16 storage differ only in the two population buffers (same FP32 macros and

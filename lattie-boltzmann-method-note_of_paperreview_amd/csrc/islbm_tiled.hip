@@ -77,15 +77,24 @@ struct Lay {
     static constexpr int USED = W ? OFF_Y + 6 * Y_SZ : OFF_Z + 2 * TY * TX;
     static constexpr int NW16 = (USED / GV + 63) / 64;
     static constexpr int STAGE = W ? NW16 * 256 : USED;
-    // staged levels in LDS: a level's loads are issued two steps before it is appended
+    // 16-bit codes with 16-byte staging are decoded once per staged value, by the wave that
+    // loaded it, into an FP32 copy of the level (DEC): the filters then read FP32 like the
+    // FP32 kernel instead of converting each code at every tap that uses it
+    static constexpr bool DEC = W && VB == 2;
+    // per-target tables: NBUF slots.  Staged levels in LDS: a level's loads are issued two
+    // steps before it is appended; NRAW buffers of staged codes (a decoded level's buffer is
+    // free once its wave has decoded it, before the barrier of the step that appends it)
     static constexpr int NBUF = 3;
-    static constexpr int OFF_YP = NBUF * STAGE;  // group A: y pass of 7-10 for its row, 4 x HXW per wave
+    static constexpr int NRAW = DEC ? 2 : NBUF;
+    static constexpr int DEC_SZ = DEC ? (USED + 3) / 4 * 4 : 0;  // a decoded level, two buffers
+    static constexpr int OFF_DEC = NRAW * STAGE;
+    static constexpr int OFF_YP = OFF_DEC + 2 * DEC_SZ;  // group A: y pass of 7-10 for its row, 4 x HXW per wave
     static constexpr int OFF_HO = OFF_YP + TY * 4 * HXW;
     // per-target tables staged two steps ahead, NBUF slots (slot = hand-over step % NBUF)
     static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, NBUF x TY rows x 64
     // rows of directions 0, 1, 2 of a target level, NBUF slots x TY rows: 0 at its nodes, 1
     // and 2 with the x halo (staged by group B, read by group A)
-    static constexpr int P_ROW = W ? 256 : TX + 2 * HX;
+    static constexpr int P_ROW = W ? (VB == 2 ? 128 : 256) : TX + 2 * HX;
     static constexpr int OFF_P012 = OFF_XIB + NBUF * TY * 64;
     static constexpr int LDS_FLOATS = OFF_P012 + NBUF * TY * P_ROW;
     // workgroups resident at once on 256 CUs: the LDS-DMA staging runs at one per CU
@@ -94,7 +103,7 @@ struct Lay {
                          OFF_P012 % 4 == 0 && P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
 };
-// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32) or 112 KB (BF16, FP16): one workgroup per CU
+// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32) or 157 KB (BF16, FP16): one workgroup per CU
 static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024 &&
                   Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
@@ -300,7 +309,8 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
     constexpr int OFF_YP = LY::OFF_YP, OFF_HO = LY::OFF_HO, OFF_XIB = LY::OFF_XIB, OFF_P012 = LY::OFF_P012;
-    constexpr int P_ROW = LY::P_ROW, NBUF = LY::NBUF;
+    constexpr int P_ROW = LY::P_ROW, NBUF = LY::NBUF, NRAW = LY::NRAW;
+    constexpr bool DEC = LY::DEC;
     constexpr int TY = LY::TY, NRW = LY::NRW, HY = LY::HY, Y_SZ = LY::Y_SZ, HO_SZ = LY::HO_SZ, NW16 = LY::NW16;
     __shared__ __attribute__((aligned(16))) float lds[LY::LDS_FLOATS];
 
@@ -417,7 +427,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         // and rows 2w, 2w+1 of each y-halo direction
         auto fetch = [&](int lev, int b) {
             const unsigned lv = SZ * (unsigned)lev * (unsigned)plane;
-            float* L = lds + b * STAGE;
+            float* L = lds + b * STAGE;  // b < NRAW
             if constexpr (W) {
 #pragma unroll
                 for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m) {
@@ -461,6 +471,31 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                     const unsigned short code = (unsigned short)(__float_as_uint(R[e]) & 0xFFFFu);
                     const raw r = __builtin_bit_cast(raw, code);
                     R[e] = CD::ld(&r, 0, isc);
+                }
+            }
+        };
+        // 16-byte staging of 16-bit codes: this wave's granules of raw buffer b, 8 codes each,
+        // decoded (before their direction's scale) into decoded buffer db
+        auto decode16 = [&](int b, int db) {
+            if constexpr (DEC) {
+                const float* R = lds + b * STAGE;
+                float* D = lds + LY::OFF_DEC + db * LY::DEC_SZ;
+#pragma unroll
+                for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m) {
+                    const int n = w + NRW * m;
+                    const int G = 64 * n + lane;
+                    if (n < NW16 && G < LY::USED / LY::GV) {
+                        const uint4 c = *(const uint4*)(R + 4 * G);
+                        const unsigned cw[4] = {c.x, c.y, c.z, c.w};
+                        float v[8];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            v[2 * e] = dec16<ST>((unsigned short)(cw[e] & 0xFFFFu));
+                            v[2 * e + 1] = dec16<ST>((unsigned short)(cw[e] >> 16));
+                        }
+                        *(float4*)(D + 8 * G) = make_float4(v[0], v[1], v[2], v[3]);
+                        *(float4*)(D + 8 * G + 4) = make_float4(v[4], v[5], v[6], v[7]);
+                    }
                 }
             }
         };
@@ -515,7 +550,8 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                 }
             };
             fetch(L0, 0);
-            int b = 0;  // stage buffer of the next level to append (level L0 + a in buffer a % NBUF)
+            int b = 0;   // stage buffer of the next level to append (level L0 + a in buffer a % NRAW)
+            int db = 0;  // its decoded buffer (16-bit codes, 16-byte staging): level L0 + a in a % 2
             int s = 0;  // step
             Sched sc{L0 - 1, k0};
             bool adv;
@@ -594,8 +630,10 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                 // staged for this step are in LDS
                 wait_staged(outstanding);
                 if (adv) decode(b);
+                if (adv) decode16(b, db);
                 lds_barrier();  // level x staged by all waves; level x-1 and hand-over s-2 no longer read
-                const float* S = lds + b * STAGE;  // level x (steps that append)
+                // level x (steps that append): staged, or its decoded copy
+                const float* S = DEC ? lds + LY::OFF_DEC + db * LY::DEC_SZ : lds + b * STAGE;
                 {
                     int h1, h2;
                     lookahead(sc, h1, h2);
@@ -604,19 +642,23 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                     fetch012(h2, s2);   // read by the group A waves in step s+2
                 }
                 if (adv) {
-                    // the buffer after the next holds level x-1, filtered in the last append step
-                    const int bn = b + 2 >= NBUF ? b + 2 - NBUF : b + 2;
+                    // three raw buffers: the buffer after the next holds level x-1, filtered in
+                    // the last append step; two (decoded levels): level x's raw buffer, decoded
+                    // by every wave before the barrier above
+                    const int bn = b + 2 >= NRAW ? b + 2 - NRAW : b + 2;
                     fetch(min(x + 2, L1), bn);
-                    b = b + 1 == NBUF ? 0 : b + 1;
+                    b = b + 1 == NRAW ? 0 : b + 1;
                 }
                 outstanding = ntab + (adv ? nbatch : 0);
                 float nv[8];  // this step's filtered values of the group's directions
                 if (adv) {
                     // horizontal filters of this group's directions.
-                    // Staged value at offset o of level x: FP32 as stored, 16-bit codes
-                    // (16-byte staging) converted here; the direction's scale multiplies the
+                    // Staged value at offset o of level x: FP32 as stored, or 16-bit codes
+                    // (16-byte staging) converted to FP32 before their direction's scale (in
+                    // the decoded copy, or here where there is none); the scale multiplies the
                     // filtered value (the filters are linear).
-                    constexpr bool raw16 = W && k16;
+                    constexpr bool code16 = W && k16;
+                    constexpr bool raw16 = code16 && !DEC;
                     const unsigned short* S16 = (const unsigned short*)S;
                     auto sv = [&](int o) -> float {
                         if constexpr (raw16)
@@ -625,7 +667,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                             return S[o];
                     };
                     auto scl = [&](float v, int q) -> float {
-                        if constexpr (raw16)
+                        if constexpr (code16)
                             return v * a.inv_scale[q];
                         else
                             return v;
@@ -705,6 +747,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                             nv[2 + d] = scl(yf(OFF_Y + d * Y_SZ + ty * TX + tx, wy[ysel(qy(d))]), qy(d));
                     }
                 }
+                if (adv) db ^= 1;
                 // hand-over of target ho from the window of the last append (parity S)
                 auto handover = [&](auto wc) {
                     constexpr int S = decltype(wc)::value;

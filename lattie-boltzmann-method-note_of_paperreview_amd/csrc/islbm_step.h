@@ -292,12 +292,15 @@ __device__ inline C wall_x_y_xi(const R& f, long long base, const int* xo, const
 // costs one memory round trip for them rather than one per branch; the branch tables are
 // read through the constant address space.
 // ------------------------------------------------------------------------------------
-template <Store ST, class C>
-__device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long node, int i, int j, int k,
-                                   const int* xo, const int* yo, long long plane)
+// pop(q): a reader of population q's stored values by node index.  The tiled kernel passes
+// one over its buffer resource (32-bit offsets), so the wall code holds no 64-bit address
+// per population array in scalar registers.
+template <Store ST, class C, class P>
+__device__ inline void apply_walls_with(C (&F)[NQ], const StepArgs<ST>& a, const P& pop, long long node, int i, int j,
+                                        int k, const int* xo, const int* yo, long long plane)
 {
     if (k > 1 && k != a.nz - 1) return;
-    auto at = [&](int q) { return Codec<ST>::ld(a.fo + (long long)q * a.qs, node, a.inv_scale[q]); };
+    auto at = [&](int q) { return pop(q)(node); };
     // FP64 kernels read them where a rule uses them: held together they would raise the
     // register allocation of those kernels
     constexpr bool kBatch = !std::is_same_v<C, double>;
@@ -323,7 +326,6 @@ __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long 
         const int ib = k * a.ny + j;
         const cptr<int> bflag = (cptr<int>)a.t.bflag;
         const cptr<C> bq = (cptr<C>)a.t.bq;
-        auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
         auto W = [&](int which) { return (cptr<C>)a.t.bw + ((long long)which * nb + ib) * 7; };
         const int f3 = bflag[B3 * nb + ib], f4 = bflag[B4 * nb + ib];
         const int f15 = bflag[B15 * nb + ib], f16 = bflag[B16 * nb + ib];
@@ -362,6 +364,14 @@ __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long 
             F[16] = wall_y_xi(pop(17), (long long)i, yo, plane, W(BY16), W(BXI16));
         }
     }
+}
+
+template <Store ST, class C>
+__device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long node, int i, int j, int k,
+                                   const int* xo, const int* yo, long long plane)
+{
+    auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
+    apply_walls_with<ST>(F, a, pop, node, i, j, k, xo, yo, plane);
 }
 
 // ------------------------------------------------------------------------------------

@@ -7,7 +7,7 @@
 //   ring waves, group A and group B: a lane keeps a ring of its group's 8 directions x 7
 //       levels in registers
 //           group A: 5 6 7 8 9 10 11 12        group B: 3 4 13 14 15 16 17 18
-//       (FP16 storage: 5 and 13 swapped, see gq)
+//       (16-bit storage: 5 and 13 swapped, see gq)
 //   collision waves: they take the 19 populations of a target from LDS, apply the walls,
 //       collide (collide_pairs, evolution.h:597-679) and store; they issue no global load.
 // The roles run separate loops with one workgroup barrier per step, so each role's
@@ -22,8 +22,9 @@
 // ring.  Once the ring covers a target level's xi stencil, each ring wave applies the xi
 // weights of the target node to its 8 directions (the last step of the interpolated
 // streaming of interpolationHillISLBM.h) and hands the results to the collision waves
-// through LDS; group A also hands over directions 0, 1 and 2 (no z dependence: the node's
-// 0 and the x filters of 1 and 2 over the target level's rows, which group B stages).
+// through LDS.  One group also hands over directions 0, 1 and 2 (no z dependence: the node's
+// 0 and the x filters of 1 and 2 over the target level's rows, which group B stages): group
+// A with FP32 storage, group B with 16-bit storage (whichever balances the groups' work).
 //
 // Staging uses buffer loads straight into LDS, so no registers hold the prefetch: with
 // 16-byte staging one dwordx4 load of 64 lanes fills 64 granules of a level; with 4-byte
@@ -114,7 +115,7 @@ __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n 
 __host__ __device__ constexpr int qy(int d) { return d < 2 ? 3 + d : 13 + d; }  // 3 4 15 16 17 18
 // the 8 directions of ring group G.  SW: directions 5 and 13 swapped between the groups
 // (group A: 6 13 7-12, group B: 3 4 5 14 15-18), which balances the groups' per-step work
-// where 16-bit decoding folds into the filters' FMAs (FP16 storage)
+// with 16-bit storage (group B then also hands over 0-2)
 __host__ __device__ constexpr int gq(int G, int d, bool SW)
 {
     return G == 0 ? (SW && d == 0 ? 6 : SW && d == 1 ? 13 : 5 + d)
@@ -305,7 +306,8 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     constexpr unsigned SZ = sizeof(raw);
     using LY = Lay<W, W ? (int)SZ : 4>;
     constexpr bool k16 = (ST != Store::F32);
-    constexpr bool SW = (ST == Store::FP16);  // ring group composition (gq)
+    constexpr bool SW = k16;           // ring group composition (gq)
+    constexpr int G012 = k16 ? 1 : 0;  // the group that hands over directions 0-2
     constexpr int HXW = LY::HXW, XO = LY::XO, XY_SZ = LY::XY_SZ, X_SZ = LY::X_SZ;
     constexpr int OFF_XY = LY::OFF_XY, OFF_X = LY::OFF_X, OFF_Y = LY::OFF_Y, OFF_Z = LY::OFF_Z, STAGE = LY::STAGE;
     constexpr int OFF_YP = LY::OFF_YP, OFF_HO = LY::OFF_HO, OFF_XIB = LY::OFF_XIB, OFF_P012 = LY::OFF_P012;
@@ -764,7 +766,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                         for (int m = 1; m < 7; ++m) sum = sum + win(d, m) * wz[m];
                         HO[q * (TX * TY) + col] = sum;
                     }
-                    if constexpr (G == 0) {
+                    if constexpr (G == G012) {
                         // directions 0, 1, 2 of the target from its rows, staged two steps ahead by
                         // group B (its wait for them precedes the barrier of this step)
                         float f0, f1, f2;

@@ -43,26 +43,32 @@ namespace islbm {
 
 namespace {
 
-constexpr int TX = 64;
-constexpr int HX = TX + 6;
-
-// Tile and LDS layout.  A workgroup owns a TX x TY tile of node columns and has 3 TY waves:
-// ring waves of group A (w = 0 .. TY-1, one per row), of group B (w = TY .. 2TY-1) and
-// collision waves (w = 2TY .. 3TY-1).  Waves w, w + 4, w + 8 share a SIMD.
-//   W = false (4-byte staging, TY = 2): rows with an x halo are 70 wide from x = i0 - 3.
-//   W = true (16-byte staging, FP32 storage with nx % 4 == 0, TY = 4): rows with an x halo
-//       are 72 wide from x = i0 - 4 (whole 16-byte granules); each SIMD then holds one
-//       wave of each role.  A level is staged by NW16 dwordx4 LDS-DMA loads of 64 lanes,
-//       lane l of load n filling granule 64 n + l (16 bytes at float 4 (64 n + l)).
+// Tile and LDS layout.  A workgroup owns a TX x TY tile of node columns and has 3 NWR waves:
+// ring waves of group A (w = 0 .. NWR-1), of group B (w = NWR .. 2NWR-1) and collision waves
+// (w = 2NWR .. 3NWR-1).  A wave covers R = 64 / TX rows of the tile: lane l takes column
+// l % TX of row R (w % NWR) + l / TX.  Waves w, w + 4, w + 8 share a SIMD.
+//   W = false (4-byte staging, TX = 64, TY = 2): rows with an x halo are 70 wide from
+//       x = i0 - 3.
+//   W = true (16-byte staging, TY = 4 R): rows with an x halo are TX + 8 wide from x = i0 - 4
+//       (whole 16-byte granules); each SIMD then holds one wave of each role.  A level is
+//       staged by NW16 dwordx4 LDS-DMA loads of 64 lanes, lane l of load n filling granule
+//       64 n + l (16 bytes at float 4 (64 n + l)).  TX = 64 (one row per wave), or TX = 32
+//       (FP32, two rows per wave) for grids whose rows fill 32-column tiles but not 64-column
+//       ones (nx = 32, 96, ...), where a 64-column tile would leave lanes idle.
 // A staged level holds
 //   XY: 7-10 [4][HY][HXW]   X: 11-14 [4][TY][HXW]   Z: 5, 6 [2][TY][TX]   Y: 3 4 15-18 [6][HY][TX]
 // (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).
-template <bool W, int VB = 4>  // VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16)
+template <bool W, int VB = 4, int TXW = 64>  // VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16)
 struct Lay {
+    static_assert(TXW == 64 || (W && VB == 4 && TXW == 32), "32-column tiles: FP32 with 16-byte staging only");
+    static constexpr int TX = TXW;
+    static constexpr int HX = TX + 6;
+    static constexpr int R = 64 / TX;    // rows per wave
     static constexpr int GV = 16 / VB;  // values per 16-byte granule
-    static constexpr int TY = W ? 4 : 2;
-    static constexpr int NRW = 2 * TY;  // ring waves
-    static constexpr int NCW = TY;      // collision waves
+    static constexpr int TY = W ? 4 * R : 2;
+    static constexpr int NWR = W ? 4 : 2;  // waves per role
+    static constexpr int NRW = 2 * NWR;    // ring waves
+    static constexpr int NCW = NWR;        // collision waves
     static constexpr int NTH = 64 * (NRW + NCW);
     static constexpr int HY = TY + 6;
     static constexpr int Y_SZ = HY * TX;       // y halo only
@@ -94,8 +100,9 @@ struct Lay {
     // per-target tables staged two steps ahead, NBUF slots (slot = hand-over step % NBUF)
     static constexpr int OFF_XIB = OFF_HO + 2 * HO_SZ;  // xi weights of a target, NBUF x TY rows x 64
     // rows of directions 0, 1, 2 of a target level, NBUF slots x TY rows: 0 at its nodes, 1
-    // and 2 with the x halo (staged by group B, read by group A)
-    static constexpr int P_ROW = W ? (VB == 2 ? 128 : 256) : TX + 2 * HX;
+    // and 2 with the x halo (staged by group B, read by group A); with 16-byte staging one
+    // load stages the R rows of a wave, 64 / R lanes (granules) per row
+    static constexpr int P_ROW = W ? (VB == 2 ? 128 : 256 / R) : TX + 2 * HX;
     static constexpr int OFF_P012 = OFF_XIB + NBUF * TY * 64;
     static constexpr int LDS_FLOATS = OFF_P012 + NBUF * TY * P_ROW;
     // workgroups resident at once on 256 CUs: the LDS-DMA staging runs at one per CU
@@ -103,10 +110,13 @@ struct Lay {
     static_assert(!W || (OFF_X % GV == 0 && OFF_Z % GV == 0 && OFF_Y % GV == 0 && STAGE % 4 == 0 &&
                          OFF_P012 % 4 == 0 && P_ROW % 4 == 0), "16-byte staging needs granule-aligned regions");
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
+    static_assert(!W || P_ROW >= 4 * (TX / GV + 2 * (HXW / GV)), "rows of 0-2 exceed their slot");
 };
-// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32) or 157 KB (BF16, FP16): one workgroup per CU
+// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32), 143 KB (FP32, 32-column tiles) or
+// 157 KB (BF16, FP16): one workgroup per CU
 static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024 &&
-                  Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024, "LDS budget");
+                  Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true, 4, 32>::LDS_FLOATS * 4 <= 160 * 1024,
+              "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
 __host__ __device__ constexpr unsigned vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
@@ -134,10 +144,10 @@ struct TiledGeom {
     int ntx, nty, ntiles, nchunks, kc, nwork, nblocks;
 };
 
-TiledGeom tiled_geom(int nx, int ny, int nz, int ty, int resident)
+TiledGeom tiled_geom(int nx, int ny, int nz, int tx, int ty, int resident)
 {
     TiledGeom t;
-    t.ntx = (nx + TX - 1) / TX;
+    t.ntx = (nx + tx - 1) / tx;
     t.nty = (ny + ty - 1) / ty;
     t.ntiles = t.ntx * t.nty;
     // z chunks of at least 16 levels (each chunk re-reads a 6-level xi halo), as many as
@@ -250,11 +260,11 @@ __device__ inline void lds_dma16(__amdgpu_buffer_rsrc_t r, float* ldst, unsigned
 // Byte offset of staging granule G (16-byte staging) for the tile at (i0, j0): the level's
 // offset goes into the load's scalar offset.  Granules past the staged level (padding of
 // the last load) repeat granule 0.
-template <int VB>
+template <int VB, int TXW>
 __device__ inline unsigned granule_offset(int G, int i0, int j0, int nx, int ny, unsigned nbq)
 {
-    using LW = Lay<true, VB>;
-    constexpr int TY = LW::TY, HY = LW::HY, Y_SZ = LW::Y_SZ, GV = LW::GV;
+    using LW = Lay<true, VB, TXW>;
+    constexpr int TX = LW::TX, TY = LW::TY, HY = LW::HY, Y_SZ = LW::Y_SZ, GV = LW::GV;
     constexpr int RH = LW::HXW / GV, RN = TX / GV;  // granules per row with / without x halo
     auto wx = [&](int x) { return ((x % nx) + nx) % nx; };
     auto wy = [&](int y) { return ((y % ny) + ny) % ny; };
@@ -296,7 +306,7 @@ __device__ inline float dec16(unsigned short h)
 
 // One workgroup's work of the stream-collide step of one solve (bid: its block index
 // within that solve's launch).  Inlined into the single-solve kernel and the grouped one.
-template <Store ST, bool W>
+template <Store ST, bool W, int TXW = 64>
 __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<ST>& a, const TiledGeom& tg,
                                                                   const int bid)
 {
@@ -304,7 +314,8 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     using raw = typename CD::raw;
     using IO = BufIO<ST>;
     constexpr unsigned SZ = sizeof(raw);
-    using LY = Lay<W, W ? (int)SZ : 4>;
+    using LY = Lay<W, W ? (int)SZ : 4, TXW>;
+    constexpr int TX = LY::TX, HX = LY::HX, R = LY::R, NWR = LY::NWR;
     constexpr bool k16 = (ST != Store::F32);
     constexpr bool SW = k16;           // ring group composition (gq)
     constexpr int G012 = k16 ? 1 : 0;  // the group that hands over directions 0-2
@@ -335,7 +346,9 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     const int k1 = min(nz, k0 + tg.kc) - 1;
     const int L0 = xi_stencil_start(k0, nz);
     const int L1 = xi_stencil_start(k1, nz) + 6;
-    const int tx = lane;
+    // this lane's column and its row within the wave's R rows
+    const int tx = R == 1 ? lane : lane % TX;
+    const int rl = R == 1 ? 0 : lane / TX;
     const int i = i0 + tx;
     const int iw = i < nx ? i : i % nx;
     const auto rfo = rsrc_of(a.fo, NQ * nbq);
@@ -385,7 +398,8 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     if (wave < NRW) {
         // ------------------------------------------------------------------ ring waves
         const int w = wave;  // fetch share
-        const int ty = w % TY;
+        const int wr = w % NWR;
+        const int ty = R * wr + rl;  // row of this lane
         const int col = ty * TX + tx;
         const int j = j0 + ty;
         const int jw = j < ny ? j : j % ny;
@@ -407,7 +421,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         if constexpr (W) {
 #pragma unroll
             for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m)
-                g16[m] = granule_offset<W ? (int)SZ : 4>(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
+                g16[m] = granule_offset<W ? (int)SZ : 4, TX>(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
         }
         float wx[2][7], wy[2][7];
 #pragma unroll
@@ -536,7 +550,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             // until only the loads of the step before are outstanding (vmcnt = their count).
             const int nbatch = W ? (NW16 - w + NRW - 1) / NRW : (G == 0 ? 32 : 34);
             constexpr int nP012 = W ? 1 : 5;
-            constexpr int ntab = G == 1 ? 1 + nP012 : 0;  // xi + rows of 0-2, every step
+            constexpr int ntab = G == 1 ? R + nP012 : 0;  // xi of the wave's rows + rows of 0-2, every step
             auto wait_staged = [&](int outstanding) {
                 // the counts a ring wave can leave outstanding (0 waits for everything)
                 switch (outstanding) {
@@ -563,19 +577,25 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             const auto rxi = rsrc_of(a.t.xiw, xi_bytes);
             unsigned p16 = 0;  // 16-byte staging of directions 0-2 (group B): this lane's granule
             if constexpr (G == 1 && W) {
-                // granules: direction 0 at the nodes, then 1 and 2 with the x halo from i0 - GV
+                // granules: direction 0 at the nodes, then 1 and 2 with the x halo from i0 - GV;
+                // lanes TX rl .. TX rl + TX - 1 stage row rl of the wave
                 constexpr int GV = LY::GV, n0 = TX / GV, n12 = HXW / GV;
-                const int g = lane < n0 + 2 * n12 ? lane : 0;
+                const int g = tx < n0 + 2 * n12 ? tx : 0;
                 const int q = g < n0 ? 0 : (g < n0 + n12 ? 1 : 2);
                 const int x = g < n0 ? i0 + GV * g : i0 - GV + GV * (g - (q == 1 ? n0 : n0 + n12));
                 p16 = (unsigned)q * nbq + SZ * (unsigned)(jw * nx + ((x % nx) + nx) % nx);
             }
             auto fetch_xi = [&](int tk, int par) {
-                if constexpr (G == 1)  // an FP32 table whatever the population storage
-                    if (lane < NXI * 7)
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                            rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + ty) * 64), 4,
-                            (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jw) * (NXI * 7))), 0, 0);
+                if constexpr (G == 1) {  // an FP32 table whatever the population storage
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {  // one load per row of the wave
+                        const int jr = wrap1(j0 + R * wr + r, ny);
+                        if (lane < NXI * 7)
+                            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                                rxi, (__attribute__((address_space(3))) void*)(lds + OFF_XIB + (par * TY + R * wr + r) * 64),
+                                4, (int)(4u * lane), (int)(4u * (unsigned)(((long long)tk * ny + jr) * (NXI * 7))), 0, 0);
+                    }
+                }
             };
             // rows of directions 0, 1, 2 of target tk, staged by the group B wave of each row
             // for the group A hand-over (no z dependence)
@@ -585,8 +605,9 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                     // (FP32: granules 0-15, 16-33, 34-51; 16-bit: 0-7, 8-17, 18-27); the
                     // remaining lanes stay idle
                     constexpr int n012 = TX / LY::GV + 2 * (HXW / LY::GV);  // granules of the three rows
-                    if (lane < n012)
-                        lds_dma16(rfo, lds + OFF_P012 + (par * TY + ty) * P_ROW, p16, SZ * (unsigned)tk * (unsigned)plane);
+                    static_assert(4 * TX == P_ROW || R == 1, "lanes of row rl land at row rl's slot");
+                    if (tx < n012)
+                        lds_dma16(rfo, lds + OFF_P012 + (par * TY + R * wr) * P_ROW, p16, SZ * (unsigned)tk * (unsigned)plane);
                 } else if constexpr (G == 1) {
                     const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
                     float* P = lds + OFF_P012 + (par * TY + ty) * P_ROW;
@@ -701,7 +722,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
 #pragma unroll
                         for (int d = 0; d < 4; ++d) {
                             const float* wyy = wy[ysel(7 + d)];
-                            const int c = XO + lane;
+                            const int c = XO + tx;
                             const int o = OFF_XY + d * XY_SZ + ty * HXW + c;
                             float sum = sv(o) * wyy[0];
 #pragma unroll
@@ -709,15 +730,16 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                             YP[d * HXW + c] = sum;
                         }
                         static_assert(ysel(7) == ysel(8) && ysel(9) == ysel(10), "7, 8 and 9, 10 share y weights");
-                        if (lane < 24) {
-                            const int d = lane / 6, c = XO + 64 + lane % 6;
+                        if (lane < 24 * R) {  // row R wr + lane / 24
+                            const int e = lane % 24, row = R * wr + lane / 24;
+                            const int d = e / 6, c = XO + TX + e % 6;
                             const bool first = d < 2;
-                            const int o = OFF_XY + d * XY_SZ + ty * HXW + c;
+                            const int o = OFF_XY + d * XY_SZ + row * HXW + c;
                             float sum = sv(o) * (first ? wy[ysel(7)][0] : wy[ysel(9)][0]);
 #pragma unroll
                             for (int n = 1; n < 7; ++n)
                                 sum = sum + sv(o + n * HXW) * (first ? wy[ysel(7)][n] : wy[ysel(9)][n]);
-                            YP[d * HXW + c] = sum;
+                            lds[OFF_YP + row * 4 * HXW + d * HXW + c] = sum;
                         }
                         if constexpr (SW) {
                             nv[0] = scl(sv(OFF_Z + (TY + ty) * TX + tx), 6);                           // 6 at the node
@@ -814,7 +836,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
             lds_barrier();
         };
-        if (w < TY)
+        if (w < NWR)
             ring(std::integral_constant<int, 0>{});
         else
             ring(std::integral_constant<int, 1>{});
@@ -822,7 +844,8 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         // ------------------------------------------------------------- collision waves
         // In step s they collide the target handed over in step s-1, while the ring waves
         // filter and hand over the next one.
-        const int ty = wave - NRW;
+        const int cw = wave - NRW;
+        const int ty = R * cw + rl;  // row of this lane
         const int col = ty * TX + tx;
         const int j = j0 + ty;
         const int jw = j < ny ? j : j % ny;
@@ -848,9 +871,11 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         // Lanes outside the grid (partial tiles) carry weight 0.  The global loads below
         // precede every store of the wave.
         const bool probe = a.flux_part != nullptr && j == a.flux_plane;
+        // a wave holding the probe row (its other rows' lanes add zeros)
+        const bool probe_wave = a.flux_part != nullptr && a.flux_plane >= j0 + R * cw && a.flux_plane < j0 + R * cw + R;
         double pdx = 0.0, psum = 0.0;
         if (probe) pdx = i < nx ? a.flux_dx[iw] : 0.0;
-        if (a.flux_part != nullptr && wk == 0 && ty == 0 && lane == 0) {
+        if (a.flux_part != nullptr && wk == 0 && cw == 0 && lane == 0) {
             // the previous step's partials (written by the previous launch), in slot order
             const double* prev = a.flux_part + (long long)(1 - a.flux_par) * a.flux_nslot;
             double t = 0.0;
@@ -882,8 +907,18 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                     apply_walls<ST>(F, a, node, iw, jw, tk, xo, yo, plane);
                 }
             }
-            const float delta = a.les ? ((const __attribute__((address_space(4))) float*)a.t.les_delta)[(long long)tk * ny + jw]
-                                      : 0.f;
+            // LES filter width of the row (scalar loads: one per row of the wave)
+            const auto les_tab = (const __attribute__((address_space(4))) float*)a.t.les_delta;
+            float delta = 0.f;
+            if (a.les) {
+                if constexpr (R == 1) {
+                    delta = les_tab[(long long)tk * ny + jw];
+                } else {
+                    const float d0 = les_tab[(long long)tk * ny + wrap1(j0 + R * cw, ny)];
+                    const float d1 = les_tab[(long long)tk * ny + wrap1(j0 + R * cw + 1, ny)];
+                    delta = rl ? d1 : d0;
+                }
+            }
             float rho, ux, uy, uz, tau_eff;
             collide_pairs<true>(F, rho_mod, (float)a.dt, force, delta, a.les, (float)a.tau, (float)a.cs, rho, ux, uy, uz,
                                 tau_eff, (float)(2.0 * a.tau - 1.0));
@@ -924,7 +959,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             pend = ho;
             ++s;
         }
-        if (probe) {
+        if (probe_wave) {
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) psum += __shfl_xor(psum, off);
             if (lane == 0) a.flux_part[(long long)a.flux_par * a.flux_nslot + chunk * tg.ntx + tile % tg.ntx] = psum;
@@ -956,7 +991,8 @@ struct GroupArgs {
 };
 static_assert(sizeof(GroupArgs<Store::F32>) <= 4096, "grouped kernel arguments exceed 4 KiB");
 
-template <Store ST, bool W>
+static_assert(Lay<true, 4, 32>::NTH == Lay<true>::NTH, "both tile widths run 12 waves");
+template <Store ST, bool W, int TXW>
 __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled_group(
     GroupArgs<ST> g)
 {
@@ -964,10 +1000,28 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
     int e = 0;
     while (e + 1 < g.n && b >= g.block0[e + 1]) ++e;
     if (g.step >= g.limit[e] || b - g.block0[e] >= g.tg[e].nblocks) return;
-    tiled_body<ST, W>(g.a[e], g.tg[e], b - g.block0[e]);
+    tiled_body<ST, W, TXW>(g.a[e], g.tg[e], b - g.block0[e]);
 }
 
 bool tiled_wide_staging(const LaunchGeom& g, Store store);
+
+// Tile width of an FP32 solve with 16-byte staging: 32 columns where that leaves fewer
+// lanes idle than 64-column tiles (nx = 16, 20, 32, 80, 96, ...), else 64.  Fixed by the
+// grid, so a solve always runs the same compiled kernel.
+int tiled_tile_width(const LaunchGeom& g, Store store)
+{
+    if (store != Store::F32 || !tiled_wide_staging(g, store)) return 64;
+    const int lanes64 = (g.nx + 63) / 64 * 64, lanes32 = (g.nx + 31) / 32 * 32;
+    return lanes32 < lanes64 ? 32 : 64;
+}
+
+TiledGeom tiled_geom_of(const LaunchGeom& g, Store store)
+{
+    if (!tiled_wide_staging(g, store)) return tiled_geom(g.nx, g.ny, g.nz, 64, Lay<false>::TY, Lay<false>::RESIDENT);
+    if (tiled_tile_width(g, store) == 32)
+        return tiled_geom(g.nx, g.ny, g.nz, 32, Lay<true, 4, 32>::TY, Lay<true, 4, 32>::RESIDENT);
+    return tiled_geom(g.nx, g.ny, g.nz, 64, Lay<true>::TY, Lay<true>::RESIDENT);
+}
 
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
 {
@@ -980,9 +1034,7 @@ bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
 
 int tiled_flux_slots(const LaunchGeom& g, Store store)
 {
-    const bool wide = tiled_wide_staging(g, store);
-    const TiledGeom t = wide ? tiled_geom(g.nx, g.ny, g.nz, Lay<true>::TY, Lay<true>::RESIDENT)
-                             : tiled_geom(g.nx, g.ny, g.nz, Lay<false>::TY, Lay<false>::RESIDENT);
+    const TiledGeom t = tiled_geom_of(g, store);
     return t.nchunks * t.ntx;
 }
 
@@ -990,7 +1042,7 @@ template <Store ST, bool W>
 static hipError_t launch_t(hipStream_t st, const StepLaunch& L)
 {
     StepArgs<ST> a = make_step_args<ST>(L);
-    TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz, Lay<W>::TY, Lay<W>::RESIDENT);
+    TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz, 64, Lay<W>::TY, Lay<W>::RESIDENT);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W>), dim3(tg.nblocks), dim3(Lay<W>::NTH), 0, st, a, tg);
     return hipGetLastError();
 }
@@ -1007,29 +1059,44 @@ bool tiled_groupable(const LaunchGeom& g, Store store)
     return store == Store::F32 && tiled_supported(g, store, false) && tiled_wide_staging(g, store);
 }
 
+// The members of one tile width go into one launch: a group of mixed widths takes two
+// launches per step (64-column members first).
 hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* limit, int n, int step)
 {
     if (n <= 0 || n > kGroupMax) return hipErrorInvalidValue;
-    GroupArgs<Store::F32> g{};
-    int blocks = 0;
     for (int e = 0; e < n; ++e) {
         if (L[e].store != Store::F32 || !tiled_wide_staging(L[e].g, L[e].store)) return hipErrorInvalidValue;
         const size_t fb = (size_t)L[e].g.n * 4;
         const char* r = (const char*)L[e].rho;
         if ((const char*)L[e].u != r + fb || (const char*)L[e].v != r + 2 * fb || (const char*)L[e].w != r + 3 * fb)
             return hipErrorInvalidValue;
-        g.a[e] = make_step_args<Store::F32>(L[e]);
-        g.tg[e] = tiled_geom(L[e].g.nx, L[e].g.ny, L[e].g.nz, Lay<true>::TY, Lay<true>::RESIDENT);
-        g.block0[e] = blocks;
-        g.limit[e] = limit[e];
-        blocks += g.tg[e].nblocks;  // a multiple of 8
     }
-    g.block0[n] = blocks;
-    g.n = n;
-    g.step = step;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled_group<Store::F32, true>), dim3(blocks), dim3(Lay<true>::NTH), 0,
-                       st, g);
-    return hipGetLastError();
+    for (const int width : {64, 32}) {
+        GroupArgs<Store::F32> g{};
+        int blocks = 0, m = 0;
+        for (int e = 0; e < n; ++e) {
+            if (tiled_tile_width(L[e].g, L[e].store) != width) continue;
+            g.a[m] = make_step_args<Store::F32>(L[e]);
+            g.tg[m] = tiled_geom_of(L[e].g, L[e].store);
+            g.block0[m] = blocks;
+            g.limit[m] = limit[e];
+            blocks += g.tg[m].nblocks;  // a multiple of 8
+            ++m;
+        }
+        if (m == 0) continue;
+        g.block0[m] = blocks;
+        g.n = m;
+        g.step = step;
+        if (width == 64)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled_group<Store::F32, true, 64>), dim3(blocks),
+                               dim3(Lay<true>::NTH), 0, st, g);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled_group<Store::F32, true, 32>), dim3(blocks),
+                               dim3(Lay<true, 4, 32>::NTH), 0, st, g);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)

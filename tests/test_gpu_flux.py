@@ -13,7 +13,8 @@ KINDS = [dict(nx=64, ny=32, nz=48, precision="fp32"),        # tiled, 16-byte st
          dict(nx=16, ny=32, nz=32, precision="fp32"),        # tiled, one x tile wider than the grid
          dict(nx=48, ny=21, nz=40, precision="bf16", cs_smag=0.1),  # partial row tile
          dict(nx=18, ny=20, nz=24, precision="fp16"),        # tiled, 4-byte staging
-         dict(nx=160, ny=36, nz=70, precision="fp32"),       # three x tiles, several z chunks
+         dict(nx=160, ny=36, nz=70, precision="fp32"),       # five 32-column tiles, several z chunks
+         dict(nx=32, ny=35, nz=30, precision="fp32"),        # 32-column tiles, the probe row first in its wave
          dict(nx=16, ny=32, nz=32, precision="fp64")]        # two-pass FP64, separate probe
 
 

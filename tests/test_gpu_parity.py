@@ -271,11 +271,14 @@ ALPHA08 = dict(hill_alpha=0.8, ly=5.142 + 3.858 * 0.8, ny=48, nz=32)
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
 @pytest.mark.parametrize("shape", [dict(nx=64, ny=32, nz=32), dict(nx=48, ny=20, nz=24), dict(nx=16, ny=32, nz=64),
                                    dict(nx=64, ny=32, nz=48, cs_smag=0.1), dict(nx=20, ny=21, nz=24),
-                                   dict(nx=18, ny=20, nz=24), dict(nx=16, **ALPHA08), dict(nx=18, **ALPHA08)])
+                                   dict(nx=18, ny=20, nz=24), dict(nx=16, **ALPHA08), dict(nx=18, **ALPHA08),
+                                   dict(nx=32, ny=20, nz=24), dict(nx=96, ny=20, nz=32, cs_smag=0.1)])
 def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
     """Column-marching kernel (z chunks, wrapped partial tiles) against the FP64 oracle.
     FP32 storage with nx % 4 == 0 stages whole 16-byte granules (nx = 16, 20, 48 wrap inside
-    a tile; ny = 21 leaves a partial row tile); nx = 18 takes the 4-byte staging.  The
+    a tile; ny = 21 leaves a partial row tile); nx = 18 takes the 4-byte staging.  FP32 grids
+    with nx = 16, 20, 32 and 96 take 32-column tiles of 8 rows (two rows per wave; ny = 20
+    leaves a partial row tile), the others 64-column tiles.  The
     alpha = 0.8 hill has BFL nodes of both branches in the 3/4 and 15/16 families, so the
     interpolated wall branch (q < 1/2) runs on the tiled kernels too."""
     o, s = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED, **shape)

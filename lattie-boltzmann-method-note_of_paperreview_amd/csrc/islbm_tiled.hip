@@ -2,8 +2,9 @@
 // FP32 and 16-bit population storage (FP32 arithmetic, deviation form f - w_q).
 //
 // Work decomposition.  A workgroup owns a 64 (x) by TY (y) tile of node columns (TY = 4 with
-// 16-byte staging, 2 with 4-byte staging) and marches up a chunk of z levels.  It has 3 TY
-// waves in three roles, one wave of each role per y row:
+// 16-byte staging, 2 with 4-byte staging; FP32 grids that fill 32-column tiles better take
+// 32 x 8 tiles, two rows per wave) and marches up a chunk of z levels.  It has 3 x 4 (or
+// 3 x 2) waves in three roles, one wave of each role per y row (per row pair):
 //   ring waves, group A and group B: a lane keeps a ring of its group's 8 directions x 7
 //       levels in registers
 //           group A: 5 6 7 8 9 10 11 12        group B: 3 4 13 14 15 16 17 18

@@ -31,7 +31,11 @@ BYTES_PER_LUP = {"fp64": 38 * 8 + 4 * 8, "fp32": 38 * 4 + 4 * 4, "bf16": 38 * 2 
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS + % HBM roofline (FP32 vs FP64 @ stated err); serving p50/p99 solve latency"
 
-GRID = dict(nx=128, ny=256, nz=128)
+GRID = dict(nx=128, ny=256, nz=128)  # --grid changes it (other shapes are diagnostic runs)
+
+
+def grid_tag() -> str:
+    return f"{GRID['nx']}x{GRID['ny']}x{GRID['nz']}"
 PHYSICS = {
     # BASELINE's production point: Re_h 2800, U_b 0.05, Smagorinsky Cs 0.1, tanh stretch 2.0,
     # with the opt-in near-wall xi treatment (the reference's xi weights diverge here)
@@ -52,7 +56,7 @@ PHYSICS = {
 def traffic_from_profiles(kernel_tag: str, prec: str):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled
     by the copy-kernel calibration), committed under profiles/; None if not measured."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}_{prec}_128x256x128.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}_{prec}_{grid_tag()}.json")
     if not os.path.exists(path):
         return None
     try:
@@ -71,6 +75,15 @@ def kernel_name(ktag: str, prec: str) -> str:
     return "k_collide_pernode"
 
 
+def tile_columns(prec: str) -> int:
+    """Tile width the library picks for an FP32 grid with 16-byte staging: 32 columns where
+    that leaves fewer lanes idle than 64 (two rows per wave, 32 x 8 tiles), else 64 (64 x 4)."""
+    nx = GRID["nx"]
+    if prec != "fp32" or nx % 4:
+        return 64
+    return 32 if (nx + 31) // 32 * 32 < (nx + 63) // 64 * 64 else 64
+
+
 def cpu_baseline(physics: dict, steps: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -87,7 +100,7 @@ def cpu_baseline(physics: dict, steps: int):
     n = GRID["nx"] * GRID["ny"] * GRID["nz"]
     o.close()
     return {"value": round(n * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, {steps} steps of the same 128x256x128 workload ({dt:.1f} s)"}
+            "sample": f"FP64 oracle, {steps} steps of the same {grid_tag()} workload ({dt:.1f} s)"}
 
 
 def accuracy_vs_fp64(pkg, prec: str, physics: dict, steps: int):
@@ -208,7 +221,9 @@ def main() -> int:
     ap.add_argument("--serving-compare", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--dropin", type=int, default=0, help="also time N Launch_CollisionStreaming calls (FP64)")
+    ap.add_argument("--grid", default="128,256,128", help="NX,NY,NZ (spanwise, streamwise, wall-normal)")
     args = ap.parse_args()
+    GRID.update(zip(("nx", "ny", "nz"), (int(v) for v in args.grid.split(","))))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures the one-GPU path only; this build has no multi-GPU decomposition "
               "(run with --gpus 1)", file=sys.stderr)
@@ -260,7 +275,7 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
                                                           "fp16": "f16-storage/f32"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": "periodic hill ISLBM D3Q19 MRT, 128x256x128 (spanwise x streamwise x wall-normal)",
+        "config": {"workload": f"periodic hill ISLBM D3Q19 MRT, {grid_tag()} (spanwise x streamwise x wall-normal)",
                    "precision": args.precision, "physics": args.physics, **phys, "kernel": ktag,
                    "state_finite": finite, "timed_as": f"HIP graph chunks of {chunk} steps",
                    "eager_event_ms_per_step": round(total_ms / nev, 4)},
@@ -269,6 +284,9 @@ def main() -> int:
                      "kernel": kernel_name(ktag, args.precision), "bytes_per_lup": bpl,
                      "avg_launch_us": round(avg_collide_s * 1e6, 2)},
     }
+    if ktag == "tiled":
+        tc = tile_columns(args.precision)
+        out["roofline"]["tile"] = f"{tc}x{4 if tc == 64 else 8}"  # columns x rows of a workgroup's tile
     s.close()
     try:
         out["roofline"]["copy_gbs_measured"] = round(pkg.copy_bandwidth(1 << 30, 20), 1)

@@ -75,15 +75,6 @@ def kernel_name(ktag: str, prec: str) -> str:
     return "k_collide_pernode"
 
 
-def tile_columns(prec: str) -> int:
-    """Tile width the library picks for an FP32 grid with 16-byte staging: 32 columns where
-    that leaves fewer lanes idle than 64 (two rows per wave, 32 x 8 tiles), else 64 (64 x 4)."""
-    nx = GRID["nx"]
-    if prec != "fp32" or nx % 4:
-        return 64
-    return 32 if (nx + 31) // 32 * 32 < (nx + 63) // 64 * 64 else 64
-
-
 def cpu_baseline(physics: dict, steps: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -285,8 +276,7 @@ def main() -> int:
                      "avg_launch_us": round(avg_collide_s * 1e6, 2)},
     }
     if ktag == "tiled":
-        tc = tile_columns(args.precision)
-        out["roofline"]["tile"] = f"{tc}x{4 if tc == 64 else 8}"  # columns x rows of a workgroup's tile
+        out["roofline"]["tile"] = "{}x{}".format(*s.tile_shape())  # columns x rows of a workgroup's tile
     s.close()
     try:
         out["roofline"]["copy_gbs_measured"] = round(pkg.copy_bandwidth(1 << 30, 20), 1)

@@ -918,6 +918,20 @@ int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel)
     return ISLBM_OK;
 }
 
+int islbm_tile_shape(islbm_solver* s, int32_t* cols, int32_t* rows)
+{
+    if (!s || !cols || !rows) return fail(ISLBM_ERR_ARG, "null argument");
+    *cols = *rows = 0;
+    if (s->kernel == ISLBM_KERNEL_TILED) {
+        const LaunchGeom lg{s->p.nx, s->p.ny, s->p.nz, s->n, s->qs};
+        int c = 0, r = 0;
+        tiled_tile_shape(lg, s->store, &c, &r);
+        *cols = c;
+        *rows = r;
+    }
+    return ISLBM_OK;
+}
+
 int islbm_get_table(islbm_solver* s, const char* name, double* out, int32_t cap)
 {
     if (!s || !name || !out) return fail(ISLBM_ERR_ARG, "null argument");

@@ -129,6 +129,8 @@ int group_record(hipStream_t st, islbm_solver* const* ss, const int* limit, int 
 void group_commit(islbm_solver* const* ss, const int* limit, int n);
 // Slots of one step's probe partials written by the tiled kernel (z chunks x x tiles).
 int tiled_flux_slots(const LaunchGeom& g, Store store);
+// columns x rows of a tiled-kernel workgroup's tile for this grid and storage
+void tiled_tile_shape(const LaunchGeom& g, Store store, int* cols, int* rows);
 // Sum of the current probe-plane flux sum_k,i v dx_i dz_k into part[0] and zeros into
 // part[1 .. nslot-1]: the partials the tiled kernel's next step adds, after v was set
 // from outside the step (start, upload, restart).

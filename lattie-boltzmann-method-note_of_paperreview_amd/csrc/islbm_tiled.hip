@@ -1033,6 +1033,17 @@ bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
            4 * g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
 }
 
+void tiled_tile_shape(const LaunchGeom& g, Store store, int* cols, int* rows)
+{
+    if (!tiled_wide_staging(g, store)) {
+        *cols = 64;
+        *rows = Lay<false>::TY;
+        return;
+    }
+    *cols = tiled_tile_width(g, store);
+    *rows = *cols == 32 ? Lay<true, 4, 32>::TY : Lay<true>::TY;
+}
+
 int tiled_flux_slots(const LaunchGeom& g, Store store)
 {
     const TiledGeom t = tiled_geom_of(g, store);

@@ -26,7 +26,7 @@ HEADER_SYMBOLS = [
     "islbm_init_equilibrium", "islbm_upload_f", "islbm_download_f", "islbm_download_macros",
     "islbm_step", "islbm_sync", "islbm_modify_forcing", "islbm_get_force", "islbm_set_force",
     "islbm_set_rho_modify", "islbm_get_flux_sum", "islbm_scalars", "islbm_step_count",
-    "islbm_kernel_in_use", "islbm_get_table", "islbm_time_steps", "islbm_last_error",
+    "islbm_kernel_in_use", "islbm_tile_shape", "islbm_get_table", "islbm_time_steps", "islbm_last_error",
     "islbm_selftest_collision", "islbm_copy_bandwidth",
     "islbm_ref_init", "islbm_ref_shutdown", "islbm_ref_device_arrays",
     "Launch_CollisionStreaming", "Launch_ModifyForcingTerm",
@@ -97,6 +97,7 @@ def lib():
     L.islbm_scalars.argtypes = [vp, dp]
     L.islbm_step_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.islbm_kernel_in_use.argtypes = [vp, ctypes.POINTER(i32)]
+    L.islbm_tile_shape.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.islbm_get_table.argtypes = [vp, ctypes.c_char_p, dp, i32]
     L.islbm_time_steps.argtypes = [vp, i32, dp]
     L.islbm_last_error.restype = ctypes.c_char_p
@@ -283,6 +284,12 @@ class Solver:
         v = ctypes.c_int32()
         _check(lib().islbm_kernel_in_use(self.h, ctypes.byref(v)), "islbm_kernel_in_use")
         return v.value
+
+    def tile_shape(self) -> tuple:
+        """(columns, rows) of a tiled-kernel workgroup's tile; (0, 0) for the other kernels."""
+        c, r = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().islbm_tile_shape(self.h, ctypes.byref(c), ctypes.byref(r)), "islbm_tile_shape")
+        return c.value, r.value
 
     def table(self, name: str) -> np.ndarray:
         cap = 16

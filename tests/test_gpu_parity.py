@@ -265,6 +265,16 @@ def test_kernel_is_native(pkg):
     assert s.kernel_in_use() in (pkg.KERNEL_PERNODE, pkg.KERNEL_TILED)
 
 
+@pytest.mark.parametrize("nx,prec,shape", [(128, "fp32", (64, 4)), (96, "fp32", (32, 8)), (32, "fp32", (32, 8)),
+                                           (16, "fp32", (32, 8)), (48, "fp32", (64, 4)), (18, "fp32", (64, 2)),
+                                           (96, "bf16", (64, 4)), (96, "fp64", (0, 0))])
+def test_tile_shape_follows_the_grid(pkg, nx, prec, shape):
+    """FP32 grids whose rows fill 32-column tiles better than 64-column ones take 32 x 8 tiles;
+    rows that are not whole 16-byte granules take 64 x 2; FP64 runs the two-pass kernel."""
+    s = pkg.Solver(pkg.default_params(nx=nx, ny=32, nz=32, precision=prec))
+    assert s.tile_shape() == shape
+
+
 ALPHA08 = dict(hill_alpha=0.8, ly=5.142 + 3.858 * 0.8, ny=48, nz=32)
 
 

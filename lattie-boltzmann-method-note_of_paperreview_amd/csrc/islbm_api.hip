@@ -599,6 +599,8 @@ int islbm_step_graph(islbm_solver* s, int32_t nsteps)
     hipGraphExec_t& gx = s->graph[s->cur];
     if (!gx || s->graph_steps[s->cur] != nsteps) {
         if (gx) {
+            // a chunk launched from it may still run on the stream
+            HIPCHK(hipStreamSynchronize(s->stream));
             HIPCHK(hipGraphExecDestroy(gx));
             gx = nullptr;
             s->graph_steps[s->cur] = 0;
@@ -1082,6 +1084,15 @@ int islbm::group_limit(islbm_solver* s, int64_t want)
 }
 
 int islbm::group_parity(islbm_solver* s) { return s->cur; }
+
+int islbm::field_summary(islbm_solver* s, double* part, double out[3])
+{
+    HIPCHK(hipSetDevice(s->p.device));
+    HIPCHK(launch_field_summary(s->stream, s->fp64, s->mac[0], s->n, part, part + 3 * kSumBlocks));
+    HIPCHK(hipMemcpyAsync(out, part + 3 * kSumBlocks, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return ISLBM_OK;
+}
 
 // Enqueues max(limit) grouped launches on st (member e takes limit[e] steps) and, for a
 // member whose chunk ends on a controller step, its forcing update; host state untouched.

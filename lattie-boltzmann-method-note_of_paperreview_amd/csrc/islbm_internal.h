@@ -151,6 +151,14 @@ hipError_t launch_sum(hipStream_t st, bool fp64, const void* x, long long n, dou
 constexpr int kSumScratch = kSumBlocks + 1;
 constexpr int kMaxDevices = 64;
 hipError_t sum_scratch(int device, double** out);
+// Summary of a solve's macroscopic fields mac = rho, u, v, w ([4][n], compute precision),
+// deterministic fixed-order FP64 reductions: out = {sum rho, sum v, max |u| (NaN if any
+// component is NaN)}; part holds 3 kSumBlocks partials.
+constexpr int kSummaryScratch = 3 * kSumBlocks + 3;
+hipError_t launch_field_summary(hipStream_t st, bool fp64, const void* mac, long long n, double* part, double* out);
+// The same for a solver (islbm_api.hip): launched on its stream into part (kSummaryScratch
+// device doubles), then copied to the host; waits for the solver's stream.
+int field_summary(islbm_solver* s, double* part, double out[3]);
 
 hipError_t launch_flux_probe(hipStream_t st, const void* v, bool fp64, const double* fluxw, double* part,
                              double* acc, int nx, int ny, int nz, int jplane);

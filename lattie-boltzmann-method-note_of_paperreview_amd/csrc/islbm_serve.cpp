@@ -98,11 +98,32 @@ struct islbm_batch {
     int ghead = 0, ginflight = 0;
     hipGraphExec_t gexec = nullptr;
     std::vector<int64_t> gkey;
+    // the graph each in-flight chunk was launched from, and graphs replaced by a re-recording
+    // while chunks launched from them may still run (destroyed once none of those remains)
+    hipGraphExec_t gslot_exec[kDepth] = {nullptr, nullptr};
+    std::vector<hipGraphExec_t> gretired;
+    // device partials of a finished solve's summary (islbm::field_summary)
+    double* dsum = nullptr;
     int64_t n_group_records = 0;  // times the chunk graph was recorded
     int64_t n_group_chunks = 0;   // grouped chunks launched
 };
 
 namespace {
+
+// destroys the replaced chunk graphs that no in-flight chunk was launched from
+void reap_graphs(islbm_batch* b)
+{
+    for (size_t t = 0; t < b->gretired.size();) {
+        bool used = false;
+        for (int k = 0; k < b->ginflight; ++k) used = used || b->gslot_exec[(b->ghead + k) % kDepth] == b->gretired[t];
+        if (used) {
+            ++t;
+            continue;
+        }
+        (void)hipGraphExecDestroy(b->gretired[t]);
+        b->gretired.erase(b->gretired.begin() + (long)t);
+    }
+}
 
 void release(islbm_batch* b, Request* r)
 {
@@ -135,21 +156,19 @@ void set_failed(islbm_batch* b, Request* r, const char* why)
 int finish(islbm_batch* b, Request* r)
 {
     const size_t n = (size_t)r->p.nx * r->p.ny * r->p.nz;
-    std::vector<double> f(4 * n);
-    int rc = islbm_download_macros(r->s, f.data(), f.data() + n, f.data() + 2 * n, f.data() + 3 * n);
+    // the summary from device reductions; the fields only when the caller keeps them
+    double sum[3];
+    int rc = islbm::field_summary(r->s, b->dsum, sum);
     if (rc) return rc;
-    double sr = 0, sv = 0, umax = 0;
-    for (size_t t = 0; t < n; ++t) {
-        sr += f[t];
-        sv += f[2 * n + t];
-        const double m = std::sqrt(f[n + t] * f[n + t] + f[2 * n + t] * f[2 * n + t] + f[3 * n + t] * f[3 * n + t]);
-        umax = std::isnan(m) ? NAN : std::max(umax, m);
-    }
     r->summary[0] = (double)n;
-    r->summary[1] = sr / n;
-    r->summary[2] = umax;
-    r->summary[3] = sv / n;
-    if (r->keep) r->fields.swap(f);
+    r->summary[1] = sum[0] / n;
+    r->summary[2] = sum[2];
+    r->summary[3] = sum[1] / n;
+    if (r->keep) {
+        r->fields.resize(4 * n);
+        double* f = r->fields.data();
+        if ((rc = islbm_download_macros(r->s, f, f + n, f + 2 * n, f + 3 * n))) return rc;
+    }
     r->state = ISLBM_REQ_DONE;
     r->t_done = now_s();
     release(b, r);
@@ -203,8 +222,10 @@ int feed_group(islbm_batch* b)
         }
         if (!b->gexec || key != b->gkey) {
             if (b->gexec) {
-                (void)hipGraphExecDestroy(b->gexec);
+                // chunks launched from it may still be in flight: destroyed once they retire
+                b->gretired.push_back(b->gexec);
                 b->gexec = nullptr;
+                reap_graphs(b);
             }
             if (hipStreamBeginCapture(b->gstream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return bfail(ISLBM_ERR_HIP, "group capture");
@@ -228,6 +249,7 @@ int feed_group(islbm_batch* b)
         b->n_group_chunks++;
         const int slot = (b->ghead + b->ginflight) % kDepth;
         if (hipEventRecord(b->gev[slot], b->gstream) != hipSuccess) return bfail(ISLBM_ERR_HIP, "event record");
+        b->gslot_exec[slot] = b->gexec;
         islbm::group_commit(ss.data(), lim.data(), (int)ss.size());
         b->gmem[slot] = mem;
         b->glim[slot] = lim;
@@ -278,9 +300,10 @@ int islbm_batch_create(int32_t device, size_t hbm_budget, int32_t max_active, in
     b->chunk = chunk;
     bool ok = hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking) == hipSuccess;
     for (hipEvent_t& e : b->gev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipMalloc((void**)&b->dsum, islbm::kSummaryScratch * sizeof(double)) == hipSuccess;
     if (!ok) {
         islbm_batch_destroy(b);
-        return bfail(ISLBM_ERR_HIP, "batch stream or events");
+        return bfail(ISLBM_ERR_HIP, "batch stream, events or summary buffer");
     }
     *out = b;
     return ISLBM_OK;
@@ -296,6 +319,8 @@ int islbm_batch_destroy(islbm_batch* b)
         delete kv.second;
     }
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    for (hipGraphExec_t g : b->gretired) (void)hipGraphExecDestroy(g);
+    if (b->dsum) (void)hipFree(b->dsum);
     for (hipEvent_t e : b->gev)
         if (e) (void)hipEventDestroy(e);
     if (b->gstream) (void)hipStreamDestroy(b->gstream);
@@ -336,6 +361,7 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
 {
     if (!b) return bfail(ISLBM_ERR_ARG, "null batch");
     if (hipSetDevice(b->device) != hipSuccess) return bfail(ISLBM_ERR_HIP, "hipSetDevice");
+    std::vector<Request*> finished;
     for (int pass = 0; pass < 2; ++pass) {
         // retire completed grouped chunks, in order
         bool progressed = false;
@@ -350,10 +376,12 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
                     mem[e]->done_steps += b->glim[b->ghead][e];
             }
             b->gmem[b->ghead].clear();
+            b->gslot_exec[b->ghead] = nullptr;
             b->ghead = (b->ghead + 1) % kDepth;
             b->ginflight--;
             progressed = true;
         }
+        reap_graphs(b);
         // retire completed chunks, in order per solve
         for (size_t t = 0; t < b->active.size();) {
             Request* r = b->active[t];
@@ -369,11 +397,10 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
                 r->inflight--;
                 progressed = true;
             }
-            if (r->state == ISLBM_REQ_ACTIVE && r->done_steps == r->nsteps) {
-                int rc = finish(b, r);
-                if (rc) set_failed(b, r, islbm_last_error());
-            }
-            if (r->state != ISLBM_REQ_ACTIVE) {
+            // a solve that took all its steps leaves the active set (its slot frees; its memory
+            // stays counted until it is finished below, after the next chunks are queued)
+            if (r->state == ISLBM_REQ_ACTIVE && r->done_steps == r->nsteps) finished.push_back(r);
+            if (r->state != ISLBM_REQ_ACTIVE || r->done_steps == r->nsteps) {
                 b->active.erase(b->active.begin() + t);
                 progressed = true;
             } else {
@@ -405,6 +432,10 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
                 if (r->grouped && r->state == ISLBM_REQ_ACTIVE) set_failed(b, r, why.c_str());
             (void)rc;
         }
+        // summaries, kept fields and release of the finished solves, while the queued chunks run
+        for (Request* r : finished)
+            if (int rc = finish(b, r)) set_failed(b, r, islbm_last_error());
+        finished.clear();
         if (!wait || progressed || b->active.empty()) break;
         // block until the oldest in-flight chunk (grouped first) completes
         if (b->ginflight > 0) {

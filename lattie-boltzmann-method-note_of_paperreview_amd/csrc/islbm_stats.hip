@@ -116,6 +116,54 @@ __global__ void k_sum_final(const double* __restrict__ part, int np, double* __r
     out[0] = acc;
 }
 
+// sum rho, sum v and max |u| over fixed blocks in fixed order; a NaN speed stays NaN
+template <typename C>
+__global__ __launch_bounds__(256) void k_summary_partial(const C* __restrict__ mac, long long n, double* __restrict__ part)
+{
+    __shared__ double red[3][256];
+    const long long per = (n + gridDim.x - 1) / gridDim.x;
+    const long long b0 = (long long)blockIdx.x * per;
+    const long long b1 = b0 + per < n ? b0 + per : n;
+    double sr = 0.0, sv = 0.0, um = 0.0;
+    for (long long t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
+        const double u = (double)mac[n + t], v = (double)mac[2 * n + t], w = (double)mac[3 * n + t];
+        sr += (double)mac[t];
+        sv += v;
+        const double m = sqrt(u * u + v * v + w * w);
+        um = (m != m || um != um) ? m + um : (m > um ? m : um);
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = sv;
+    red[2][threadIdx.x] = um;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            const double a = red[2][threadIdx.x], c = red[2][threadIdx.x + w];
+            red[2][threadIdx.x] = (a != a || c != c) ? a + c : (c > a ? c : a);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) part[(long long)q * gridDim.x + blockIdx.x] = red[q][0];
+}
+
+__global__ void k_summary_final(const double* __restrict__ part, int np, double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double sr = 0.0, sv = 0.0, um = 0.0;
+    for (int t = 0; t < np; ++t) {
+        sr += part[t];
+        sv += part[np + t];
+        const double m = part[2 * np + t];
+        um = (m != m || um != um) ? m + um : (m > um ? m : um);
+    }
+    out[0] = sr;
+    out[1] = sv;
+    out[2] = um;
+}
+
 }  // namespace
 
 const char* const kStatNames[NSTAT] = {"U",     "V",     "W",     "P",     "UU",    "UV",    "UW",   "VV",   "VW",
@@ -159,6 +207,16 @@ hipError_t sum_scratch(int device, double** out)
     }
     *out = (double*)g_scratch_ptr[device];
     return hipSuccess;
+}
+
+hipError_t launch_field_summary(hipStream_t st, bool fp64, const void* mac, long long n, double* part, double* out)
+{
+    if (fp64)
+        hipLaunchKernelGGL(k_summary_partial<double>, dim3(kSumBlocks), dim3(256), 0, st, (const double*)mac, n, part);
+    else
+        hipLaunchKernelGGL(k_summary_partial<float>, dim3(kSumBlocks), dim3(256), 0, st, (const float*)mac, n, part);
+    hipLaunchKernelGGL(k_summary_final, dim3(1), dim3(64), 0, st, (const double*)part, kSumBlocks, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_sum(hipStream_t st, bool fp64, const void* x, long long n, double* part, double* out)

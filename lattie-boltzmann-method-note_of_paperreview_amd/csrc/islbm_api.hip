@@ -1096,7 +1096,7 @@ int islbm::field_summary(islbm_solver* s, double* part, double out[3])
 
 // Enqueues max(limit) grouped launches on st (member e takes limit[e] steps) and, for a
 // member whose chunk ends on a controller step, its forcing update; host state untouched.
-int islbm::group_record(hipStream_t st, islbm_solver* const* ss, const int* limit, int n)
+int islbm::group_record(hipStream_t st, islbm_solver* const* ss, const int* limit, int n, const GroupFork* fork)
 {
     if (n <= 0 || n > kGroupMaxSolves) return fail(ISLBM_ERR_ARG, "group of %d solves", n);
     int steps = 0;
@@ -1104,7 +1104,7 @@ int islbm::group_record(hipStream_t st, islbm_solver* const* ss, const int* limi
     StepLaunch L[kGroupMaxSolves];
     for (int t = 0; t < steps; ++t) {
         for (int e = 0; e < n; ++e) L[e] = make_launch(ss[e], ss[e]->cur ^ (t & 1));
-        HIPCHK(launch_tiled_group(st, L, limit, n, t));
+        HIPCHK(launch_tiled_group(st, L, limit, n, t, fork));
     }
     for (int e = 0; e < n; ++e)
         if ((ss[e]->steps + limit[e]) % ss[e]->p.ndtfrc == 0)

@@ -113,7 +113,14 @@ bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
 // tiled kernel with 16-byte staging (tiled_groupable) by one step each; solve e takes part
 // while step < limit[e].  Results equal those of launch_stream_collide for each solve.
 bool tiled_groupable(const LaunchGeom& g, Store store);
-hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* limit, int n, int step);
+// A second stream and two events for a group of mixed tile widths: the two launches of a
+// step then run concurrently (forked from st, joined back into st).
+struct GroupFork {
+    hipStream_t st2;
+    hipEvent_t fork, join;
+};
+hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* limit, int n, int step,
+                              const GroupFork* fork = nullptr);
 constexpr int kGroupMaxSolves = 8;
 // Host side of grouped stepping (islbm_api.hip), for the serving loop:
 //   group_eligible: the solve can join a group (FP32, 16-byte-staged tiled kernel, no
@@ -125,7 +132,7 @@ constexpr int kGroupMaxSolves = 8;
 bool group_eligible(islbm_solver* s);
 int group_limit(islbm_solver* s, int64_t want);
 int group_parity(islbm_solver* s);
-int group_record(hipStream_t st, islbm_solver* const* ss, const int* limit, int n);
+int group_record(hipStream_t st, islbm_solver* const* ss, const int* limit, int n, const GroupFork* fork = nullptr);
 void group_commit(islbm_solver* const* ss, const int* limit, int n);
 // Slots of one step's probe partials written by the tiled kernel (z chunks x x tiles).
 int tiled_flux_slots(const LaunchGeom& g, Store store);

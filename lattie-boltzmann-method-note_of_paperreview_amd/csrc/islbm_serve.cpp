@@ -92,6 +92,7 @@ struct islbm_batch {
     // grouped stepping: the batch's stream, a ring of in-flight chunks (members and their
     // steps per chunk), and the recorded chunk graph with the key it was recorded for
     hipStream_t gstream = nullptr;
+    islbm::GroupFork gfork{nullptr, nullptr, nullptr};  // second stream of mixed-width groups
     hipEvent_t gev[kDepth] = {nullptr, nullptr};
     std::vector<Request*> gmem[kDepth];
     std::vector<int> glim[kDepth];
@@ -229,7 +230,7 @@ int feed_group(islbm_batch* b)
             }
             if (hipStreamBeginCapture(b->gstream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return bfail(ISLBM_ERR_HIP, "group capture");
-            const int rc = islbm::group_record(b->gstream, ss.data(), lim.data(), (int)ss.size());
+            const int rc = islbm::group_record(b->gstream, ss.data(), lim.data(), (int)ss.size(), &b->gfork);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->gstream, &g);
             if (rc || e != hipSuccess) {
@@ -299,6 +300,9 @@ int islbm_batch_create(int32_t device, size_t hbm_budget, int32_t max_active, in
     b->max_active = max_active;
     b->chunk = chunk;
     bool ok = hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&b->gfork.st2, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->gfork.fork, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->gfork.join, hipEventDisableTiming) == hipSuccess;
     for (hipEvent_t& e : b->gev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMalloc((void**)&b->dsum, islbm::kSummaryScratch * sizeof(double)) == hipSuccess;
     if (!ok) {
@@ -324,6 +328,9 @@ int islbm_batch_destroy(islbm_batch* b)
     for (hipEvent_t e : b->gev)
         if (e) (void)hipEventDestroy(e);
     if (b->gstream) (void)hipStreamDestroy(b->gstream);
+    if (b->gfork.st2) (void)hipStreamDestroy(b->gfork.st2);
+    for (hipEvent_t e : {b->gfork.fork, b->gfork.join})
+        if (e) (void)hipEventDestroy(e);
     delete b;
     return ISLBM_OK;
 }

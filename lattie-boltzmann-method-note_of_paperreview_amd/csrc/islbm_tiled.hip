@@ -1072,8 +1072,10 @@ bool tiled_groupable(const LaunchGeom& g, Store store)
 }
 
 // The members of one tile width go into one launch: a group of mixed widths takes two
-// launches per step (64-column members first).
-hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* limit, int n, int step)
+// launches per step, on st one after the other, or with `fork` the 32-column one on a second
+// stream so that the two run together (the next step waits for both).
+hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* limit, int n, int step,
+                              const GroupFork* fork)
 {
     if (n <= 0 || n > kGroupMax) return hipErrorInvalidValue;
     for (int e = 0; e < n; ++e) {
@@ -1083,7 +1085,16 @@ hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* li
         if ((const char*)L[e].u != r + fb || (const char*)L[e].v != r + 2 * fb || (const char*)L[e].w != r + 3 * fb)
             return hipErrorInvalidValue;
     }
+    bool mixed = false;
+    for (int e = 1; e < n; ++e) mixed = mixed || tiled_tile_width(L[e].g, L[e].store) != tiled_tile_width(L[0].g, L[0].store);
+    const bool forked = mixed && fork != nullptr;
+    if (forked) {
+        hipError_t e = hipEventRecord(fork->fork, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(fork->st2, fork->fork, 0);
+        if (e != hipSuccess) return e;
+    }
     for (const int width : {64, 32}) {
+        const hipStream_t sw = forked && width == 32 ? fork->st2 : st;
         GroupArgs<Store::F32> g{};
         int blocks = 0, m = 0;
         for (int e = 0; e < n; ++e) {
@@ -1101,11 +1112,16 @@ hipError_t launch_tiled_group(hipStream_t st, const StepLaunch* L, const int* li
         g.step = step;
         if (width == 64)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled_group<Store::F32, true, 64>), dim3(blocks),
-                               dim3(Lay<true>::NTH), 0, st, g);
+                               dim3(Lay<true>::NTH), 0, sw, g);
         else
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled_group<Store::F32, true, 32>), dim3(blocks),
-                               dim3(Lay<true, 4, 32>::NTH), 0, st, g);
+                               dim3(Lay<true, 4, 32>::NTH), 0, sw, g);
         const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (forked) {
+        hipError_t e = hipEventRecord(fork->join, fork->st2);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, fork->join, 0);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -185,7 +185,7 @@ void free_all(islbm_solver* s)
 }
 
 template <typename C>
-int upload_table(islbm_solver* s, void** dst, const std::vector<double>& src, bool weights7 = false)
+int upload_table(void** dst, const std::vector<double>& src, bool weights7 = false)
 {
     std::vector<C> tmp(src.begin(), src.end());
     if constexpr (sizeof(C) == 4)
@@ -201,10 +201,10 @@ int upload_tables(islbm_solver* s)
 {
     const HostGeom& g = s->g;
     int rc;
-    if ((rc = upload_table<C>(s, &s->xw0, g.xw0, true)) || (rc = upload_table<C>(s, &s->xw2, g.xw2, true)) ||
-        (rc = upload_table<C>(s, &s->yw0, g.yw0, true)) || (rc = upload_table<C>(s, &s->yw2, g.yw2, true)) ||
-        (rc = upload_table<C>(s, &s->bq, g.bq)) || (rc = upload_table<C>(s, &s->bw, g.bw, true)) ||
-        (rc = upload_table<C>(s, &s->les, g.les_delta)))
+    if ((rc = upload_table<C>(&s->xw0, g.xw0, true)) || (rc = upload_table<C>(&s->xw2, g.xw2, true)) ||
+        (rc = upload_table<C>(&s->yw0, g.yw0, true)) || (rc = upload_table<C>(&s->yw2, g.yw2, true)) ||
+        (rc = upload_table<C>(&s->bq, g.bq)) || (rc = upload_table<C>(&s->bw, g.bw, true)) ||
+        (rc = upload_table<C>(&s->les, g.les_delta)))
         return rc;
     // BFL branch codes decided in FP64 (see apply_walls)
     std::vector<int> code(g.bflag.size());

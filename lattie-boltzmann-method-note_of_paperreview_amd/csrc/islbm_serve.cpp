@@ -441,7 +441,7 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
         }
         // summaries, kept fields and release of the finished solves, while the queued chunks run
         for (Request* r : finished)
-            if (int rc = finish(b, r)) set_failed(b, r, islbm_last_error());
+            if (finish(b, r) != ISLBM_OK) set_failed(b, r, islbm_last_error());
         finished.clear();
         if (!wait || progressed || b->active.empty()) break;
         // block until the oldest in-flight chunk (grouped first) completes

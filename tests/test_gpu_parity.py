@@ -304,6 +304,24 @@ def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
     compare(o, s, tol, f"tiled {prec} {shape}")
 
 
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
+@pytest.mark.parametrize("shape", [dict(nx=8, ny=8, nz=8), dict(nx=16, ny=9, nz=9), dict(nx=12, ny=13, nz=13),
+                                   dict(nx=24, ny=9, nz=11), dict(nx=8, ny=16, nz=10)])
+def test_tiled_kernel_at_its_smallest_grids(pkg, po, prec, tol, shape):
+    """The tiled kernel from its minimum grid (8 x 8 x 8) up: columns of 8-14 levels, where
+    the clamped xi stencils of the two walls overlap and a chunk is the whole column; rows
+    of 8-24 nodes wrapped several times into one tile; row tiles with one real row (ny = 9
+    in 8-row tiles).  FP32 takes 32-column tiles here; 16-bit storage takes 16-byte staging
+    for nx = 8, 16, 24 and 4-byte staging for nx = 12."""
+    o, s = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED, **shape)
+    assert s.kernel_in_use() == pkg.KERNEL_TILED
+    for x in (o, s):
+        x.init_equilibrium(u0=(0.0, 0.0583, 0.0), amp=1e-3, seed=9)
+    o.step(20)
+    s.step(20)
+    compare(o, s, tol, f"tiled {prec} {shape}")
+
+
 @pytest.mark.parametrize("nx", [40, 39])
 def test_fp64_auto_is_twopass_and_equals_pernode(pkg, nx):
     """FP64 solvers take the two-pass kernel by default; it sums the same taps as the

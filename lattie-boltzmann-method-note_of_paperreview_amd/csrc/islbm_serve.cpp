@@ -377,6 +377,8 @@ int islbm_batch_advance(islbm_batch* b, int32_t wait, int64_t out[4])
             if (q == hipErrorNotReady) break;
             const std::vector<Request*>& mem = b->gmem[b->ghead];
             for (size_t e = 0; e < mem.size(); ++e) {
+                // a member failed by an earlier chunk or by a feed error is counted once
+                if (mem[e]->state != ISLBM_REQ_ACTIVE) continue;
                 if (q != hipSuccess)
                     set_failed(b, mem[e], hipGetErrorString(q));
                 else

@@ -1208,12 +1208,12 @@ void Launch_CollisionStreaming(double* f_old[19], double* f_new[19])
         pad.p[NQ + a] = g_ref_mac[a];
         uni.p[NQ + a] = (double*)s->mac[mac_of[a]];
     }
-    hipError_t e = launch_padded_to_unique(s->stream, pad, uni, nx, ny, nz, NQ);
-    if (e == hipSuccess) {
-        PtrSet pm{}, um{};
-        for (int a = 0; a < 4; ++a) { pm.p[a] = pad.p[NQ + a]; um.p[a] = uni.p[NQ + a]; }
-        e = launch_padded_to_unique(s->stream, pm, um, nx, ny, nz, 4);
-    }
+    // the step recomputes rho, u, v, w from the populations; of the previous step's fields only
+    // v is read (by the flux probe), so only v is gathered with the 19 population arrays
+    PtrSet pg = pad, ug = uni;
+    pg.p[NQ] = pad.p[NQ + 1];
+    ug.p[NQ] = uni.p[NQ + 1];
+    hipError_t e = launch_padded_to_unique(s->stream, pg, ug, nx, ny, nz, NQ + 1);
     if (e != hipSuccess) {
         fail(ISLBM_ERR_HIP, "Launch_CollisionStreaming gather: %s", hipGetErrorString(e));
         return;

@@ -750,53 +750,54 @@ hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L)
 // Reference padded layout <-> unique nodes (FP64), for the reference-named entry points
 // ------------------------------------------------------------------------------------
 
-// padded (I,J,K) of unique (i,j,k) is (i+3, j+3, k+3); NX6 = nx+7, NY6 = ny+7, NZ6 = nz+6
-__global__ void k_padded_to_unique(PtrSet pad, PtrSet uni, int nx, int ny, int nz, int narr)
+// The padded layout is [J][K][I] with NX6 = nx + 7, NY6 = ny + 7, NZ6 = nz + 6; padded (I, J, K)
+// of unique (i, j, k) is (i + 3, j + 3, k + 3).  Both kernels move one row of one array per
+// wave (array blockIdx.y, rows 4 blockIdx.x .. 4 blockIdx.x + 3), so the index arithmetic is
+// per wave and 32-bit; a lane moves every 64th value of the row.
+__global__ __launch_bounds__(256) void k_padded_to_unique(PtrSet pad, PtrSet uni, int nx, int ny, int nz)
 {
-    const long long n = (long long)nx * ny * nz;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * narr) return;
-    const int q = (int)(t / n);
-    const long long id = t % n;
-    const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((long long)nx * ny));
+    const int row = (int)blockIdx.x * 4 + (int)(threadIdx.x / 64);  // unique row k ny + j
+    if (row >= ny * nz) return;
+    const int k = row / ny, j = row - k * ny;
     const long long NX6 = nx + 7, NZ6 = nz + 6;
-    uni.p[q][id] = pad.p[q][(long long)(j + 3) * NX6 * NZ6 + (long long)(k + 3) * NX6 + (i + 3)];
+    const double* src = pad.p[blockIdx.y] + (j + 3) * NX6 * NZ6 + (k + 3) * NX6 + 3;
+    double* dst = uni.p[blockIdx.y] + (long long)row * nx;
+    for (int i = (int)(threadIdx.x % 64); i < nx; i += 64) dst[i] = src[i];
 }
 
 // Every padded node of the interior, the periodic duplicates and the three x and y ghost
 // layers gets the unique node it is a periodic image of (what periodicSW and the halo
 // exchange leave behind, evolution.h:732-777, 980-994); z ghost layers are never read by
 // the step and are not written.
-__global__ void k_unique_to_padded(PtrSet uni, PtrSet pad, int nx, int ny, int nz, int narr)
+__global__ __launch_bounds__(256) void k_unique_to_padded(PtrSet uni, PtrSet pad, int nx, int ny, int nz)
 {
-    const long long NX6 = nx + 7, NY6 = ny + 7, NZ6 = nz + 6;
-    const long long per = NX6 * NY6 * nz;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * narr) return;
-    const int q = (int)(t / per);
-    const long long r = t % per;
-    const int I = (int)(r % NX6);
-    const int J = (int)((r / NX6) % NY6);
-    const int k = (int)(r / (NX6 * NY6));
-    const int i = wrap_index(I - 3, nx), j = wrap_index(J - 3, ny);
-    pad.p[q][(long long)J * NX6 * NZ6 + (long long)(k + 3) * NX6 + I] = uni.p[q][((long long)k * ny + j) * nx + i];
+    const int NX6 = nx + 7, NY6 = ny + 7;
+    const int row = (int)blockIdx.x * 4 + (int)(threadIdx.x / 64);  // padded row k NY6 + J
+    if (row >= NY6 * nz) return;
+    const int k = row / NY6, J = row - k * NY6;
+    const int j = wrap_index(J - 3, ny);
+    const double* src = uni.p[blockIdx.y] + ((long long)k * ny + j) * nx;
+    double* dst = pad.p[blockIdx.y] + (long long)J * NX6 * (nz + 6) + (long long)(k + 3) * NX6;
+    for (int I = (int)(threadIdx.x % 64); I < NX6; I += 64) dst[I] = src[wrap_index(I - 3, nx)];
 }
 
 hipError_t launch_padded_to_unique(hipStream_t st, const PtrSet& pad, const PtrSet& uni, int nx, int ny, int nz,
                                    int narr)
 {
-    const long long tot = (long long)nx * ny * nz * narr;
-    hipLaunchKernelGGL(k_padded_to_unique, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, pad, uni, nx, ny,
-                       nz, narr);
+    const long long rows = (long long)ny * nz;
+    if (narr <= 0 || narr > NQ + 4 || rows > 0x7FFFFFFFLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_padded_to_unique, dim3((unsigned)((rows + 3) / 4), (unsigned)narr), dim3(256), 0, st, pad,
+                       uni, nx, ny, nz);
     return hipGetLastError();
 }
 
 hipError_t launch_unique_to_padded(hipStream_t st, const PtrSet& uni, const PtrSet& pad, int nx, int ny, int nz,
                                    int narr)
 {
-    const long long tot = (long long)(nx + 7) * (ny + 7) * nz * narr;
-    hipLaunchKernelGGL(k_unique_to_padded, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, uni, pad, nx, ny,
-                       nz, narr);
+    const long long rows = (long long)(ny + 7) * nz;
+    if (narr <= 0 || narr > NQ + 4 || rows > 0x7FFFFFFFLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_unique_to_padded, dim3((unsigned)((rows + 3) / 4), (unsigned)narr), dim3(256), 0, st, uni,
+                       pad, nx, ny, nz);
     return hipGetLastError();
 }
 

@@ -149,6 +149,34 @@ def serving(pkg, nreq: int, nsteps: int, ncmp: int):
     return out
 
 
+def stream_events(pkg, s):
+    """The HIP runtime, the solver's stream and two events for timing a region on that stream."""
+    import ctypes as ct
+    hip = ct.CDLL("libamdhip64.so")
+    hip.hipEventCreate.argtypes = [ct.POINTER(ct.c_void_p)]
+    hip.hipEventRecord.argtypes = [ct.c_void_p, ct.c_void_p]
+    hip.hipEventSynchronize.argtypes = [ct.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [ct.POINTER(ct.c_float), ct.c_void_p, ct.c_void_p]
+    hip.hipEventDestroy.argtypes = [ct.c_void_p]
+    st = ct.c_void_p()
+    assert pkg.lib().islbm_stream(s.h, ct.byref(st)) == 0, "islbm_stream"
+    ev = [ct.c_void_p(), ct.c_void_p()]
+    for e in ev:
+        assert hip.hipEventCreate(ct.byref(e)) == 0, "hipEventCreate"
+    return hip, st, ev
+
+
+def event_ms(hip, ev) -> float:
+    """Milliseconds from ev[0] to ev[1] (both recorded); destroys the events."""
+    import ctypes as ct
+    assert hip.hipEventSynchronize(ev[1]) == 0
+    ms = ct.c_float()
+    assert hip.hipEventElapsedTime(ct.byref(ms), ev[0], ev[1]) == 0
+    for e in ev:
+        hip.hipEventDestroy(e)
+    return float(ms.value)
+
+
 def dropin_timing(pkg, calls: int):
     """Launch_CollisionStreaming (the reference-named entry point, evolution.h:842) on the
     reference's padded FP64 layout at the bench shape: 19 + 19 caller-owned device arrays
@@ -199,7 +227,10 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
+    # the GPU's clocks ramp up over the first ~250 launches of this kernel under load (per-launch
+    # time 170-178 us falling to ~151 us over ~40 ms): 500 untimed steps bring it to the rate a
+    # long run sustains
+    ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
     ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled", "twopass"])
     ap.add_argument("--physics", default="production-clamped", choices=sorted(PHYSICS))
@@ -240,25 +271,35 @@ def main() -> int:
         s.step_graph(chunk, sync=False)
     s.step(args.warmup - args.warmup % 2 - (args.warmup // chunk) * chunk, sync=False)
     s.sync()
+    hip, st, ev = stream_events(pkg, s)
     t0 = time.perf_counter()
+    assert hip.hipEventRecord(ev[0], st) == 0
     for _ in range(args.steps // chunk):
         s.step_graph(chunk, sync=False)
     s.step(args.steps % chunk, sync=False)
+    assert hip.hipEventRecord(ev[1], st) == 0
     s.sync()
     wall = time.perf_counter() - t0
-    # the stream-collide kernel alone: HIP events recorded around each launch on the
-    # solver's stream over eagerly launched steps (its duration does not depend on how it
-    # is launched)
+    region_ms = event_ms(hip, ev)
+    # HIP events recorded around each stream-collide launch on the solver's stream, over
+    # eagerly launched steps (each pair also times its own event packets)
     nev = max(20, min(200, args.steps))
     total_ms, collide_ms = s.time_steps(nev)
     s.sync()
     mac = s.macros()
     finite = bool(all(np.isfinite(a).all() for a in mac))
     value = n * args.steps / wall / 1e6
-    avg_collide_s = collide_ms / 1e3 / nev
+    ktag = {pkg.KERNEL_TILED: "tiled", pkg.KERNEL_TWOPASS: "twopass"}.get(s.kernel_in_use(), "pernode")
+    if ktag == "tiled":
+        # one launch per step (the flux probe is fused; no forcing update falls in the region):
+        # the kernel's average launch is the timed region's event time over its steps
+        avg_collide_s = region_ms / 1e3 / args.steps
+        launch_timing = "HIP events around the timed region on the solver's stream, per step (one launch per step)"
+    else:
+        avg_collide_s = collide_ms / 1e3 / nev
+        launch_timing = f"HIP events around each of {nev} eager stream-collide launches"
     bpl = BYTES_PER_LUP[args.precision]
     achieved = bpl * n / avg_collide_s / 1e9
-    ktag = {pkg.KERNEL_TILED: "tiled", pkg.KERNEL_TWOPASS: "twopass"}.get(s.kernel_in_use(), "pernode")
     traffic = traffic_from_profiles(ktag, args.precision)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
@@ -273,7 +314,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel_name(ktag, args.precision), "bytes_per_lup": bpl,
-                     "avg_launch_us": round(avg_collide_s * 1e6, 2)},
+                     "avg_launch_us": round(avg_collide_s * 1e6, 2), "launch_timing": launch_timing,
+                     "eager_event_launch_us": round(collide_ms * 1e3 / nev, 2)},
     }
     if ktag == "tiled":
         out["roofline"]["tile"] = "{}x{}".format(*s.tile_shape())  # columns x rows of a workgroup's tile

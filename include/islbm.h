@@ -203,8 +203,8 @@ int islbm_step_count(islbm_solver* s, int64_t* steps);
 /* Kernel in use (ISLBM_KERNEL_PERNODE, ISLBM_KERNEL_TILED or ISLBM_KERNEL_TWOPASS). */
 int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel);
 /* Columns x rows of node columns one workgroup of the tiled kernel owns: 64 x 4, 32 x 8
- * (FP32 grids whose rows fill 32-column tiles better than 64-column ones) or 64 x 2 (rows
- * that are not whole 16-byte granules); 0 x 0 for the other kernels.  Fixed by the grid.
+ * (grids whose rows fill 32-column tiles better than 64-column ones) or 64 x 2 (rows that
+ * are not whole 16-byte granules); 0 x 0 for the other kernels.  Fixed by the grid.
  * No counterpart in the reference (its step launches one thread per node in NT = 32 thread
  * rows, evolution.h:853-854 and 940, variables.h:102). */
 int islbm_tile_shape(islbm_solver* s, int32_t* cols, int32_t* rows);

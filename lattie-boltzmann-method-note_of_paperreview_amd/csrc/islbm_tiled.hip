@@ -61,7 +61,7 @@ namespace {
 // (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).
 template <bool W, int VB = 4, int TXW = 64>  // VB: bytes of a staged value (16-byte staging: 4 FP32, 2 BF16/FP16)
 struct Lay {
-    static_assert(TXW == 64 || (W && VB == 4 && TXW == 32), "32-column tiles: FP32 with 16-byte staging only");
+    static_assert(TXW == 64 || (W && TXW == 32), "32-column tiles: 16-byte staging only");
     static constexpr int TX = TXW;
     static constexpr int HX = TX + 6;
     static constexpr int R = 64 / TX;    // rows per wave
@@ -113,10 +113,11 @@ struct Lay {
     static_assert(W || TY == 2, "the 4-byte fetch split assumes two rows per tile");
     static_assert(!W || P_ROW >= 4 * (TX / GV + 2 * (HXW / GV)), "rows of 0-2 exceed their slot");
 };
-// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32), 143 KB (FP32, 32-column tiles) or
-// 157 KB (BF16, FP16): one workgroup per CU
+// 99 KB (4-byte staging), 157 KB (16-byte staging, FP32), 143 KB (FP32, 32-column tiles),
+// 157 KB (BF16, FP16) or 154 KB (BF16, FP16, 32-column tiles): one workgroup per CU
 static_assert(Lay<false>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true>::LDS_FLOATS * 4 <= 160 * 1024 &&
-                  Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true, 4, 32>::LDS_FLOATS * 4 <= 160 * 1024,
+                  Lay<true, 2>::LDS_FLOATS * 4 <= 160 * 1024 && Lay<true, 4, 32>::LDS_FLOATS * 4 <= 160 * 1024 &&
+                  Lay<true, 2, 32>::LDS_FLOATS * 4 <= 160 * 1024,
               "LDS budget");
 // cache policy of the result stores: non-temporal, streamed past the caches (sc1 nt)
 constexpr int kStoreAux = 18;
@@ -968,11 +969,11 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     }
 }
 
-template <Store ST, bool W>
+template <Store ST, bool W, int TXW>
 __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled(StepArgs<ST> a,
                                                                                                           TiledGeom tg)
 {
-    tiled_body<ST, W>(a, tg, (int)blockIdx.x);
+    tiled_body<ST, W, TXW>(a, tg, (int)blockIdx.x);
 }
 
 // Several solves advanced by one launch (one step of each): solve e owns blocks
@@ -992,7 +993,8 @@ struct GroupArgs {
 };
 static_assert(sizeof(GroupArgs<Store::F32>) <= 4096, "grouped kernel arguments exceed 4 KiB");
 
-static_assert(Lay<true, 4, 32>::NTH == Lay<true>::NTH, "both tile widths run 12 waves");
+static_assert(Lay<true, 4, 32>::NTH == Lay<true>::NTH && Lay<true, 2, 32>::NTH == Lay<true>::NTH,
+              "both tile widths run 12 waves");
 template <Store ST, bool W, int TXW>
 __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled_group(
     GroupArgs<ST> g)
@@ -1006,12 +1008,12 @@ __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))
 
 bool tiled_wide_staging(const LaunchGeom& g, Store store);
 
-// Tile width of an FP32 solve with 16-byte staging: 32 columns where that leaves fewer
-// lanes idle than 64-column tiles (nx = 16, 20, 32, 80, 96, ...), else 64.  Fixed by the
-// grid, so a solve always runs the same compiled kernel.
+// Tile width of a solve with 16-byte staging: 32 columns where that leaves fewer lanes idle
+// than 64-column tiles (nx = 16, 32, 80, 96, ..., and for FP32 also 20, 36, ...), else 64.
+// Fixed by the grid, so a solve always runs the same compiled kernel.
 int tiled_tile_width(const LaunchGeom& g, Store store)
 {
-    if (store != Store::F32 || !tiled_wide_staging(g, store)) return 64;
+    if (!tiled_wide_staging(g, store)) return 64;
     const int lanes64 = (g.nx + 63) / 64 * 64, lanes32 = (g.nx + 31) / 32 * 32;
     return lanes32 < lanes64 ? 32 : 64;
 }
@@ -1050,12 +1052,13 @@ int tiled_flux_slots(const LaunchGeom& g, Store store)
     return t.nchunks * t.ntx;
 }
 
-template <Store ST, bool W>
+template <Store ST, bool W, int TXW = 64>
 static hipError_t launch_t(hipStream_t st, const StepLaunch& L)
 {
     StepArgs<ST> a = make_step_args<ST>(L);
-    TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz, 64, Lay<W>::TY, Lay<W>::RESIDENT);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W>), dim3(tg.nblocks), dim3(Lay<W>::NTH), 0, st, a, tg);
+    using LY = Lay<W, W ? (ST == Store::F32 ? 4 : 2) : 4, TXW>;
+    TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz, TXW, LY::TY, LY::RESIDENT);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W, TXW>), dim3(tg.nblocks), dim3(LY::NTH), 0, st, a, tg);
     return hipGetLastError();
 }
 
@@ -1145,9 +1148,13 @@ hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
         return launch_t<Store::F32, false>(st, L);
     }
     case Store::BF16:
-        return tiled_wide_staging(L.g, L.store) ? launch_t<Store::BF16, true>(st, L) : launch_t<Store::BF16, false>(st, L);
+        if (!tiled_wide_staging(L.g, L.store)) return launch_t<Store::BF16, false>(st, L);
+        return tiled_tile_width(L.g, L.store) == 32 ? launch_t<Store::BF16, true, 32>(st, L)
+                                                    : launch_t<Store::BF16, true>(st, L);
     case Store::FP16:
-        return tiled_wide_staging(L.g, L.store) ? launch_t<Store::FP16, true>(st, L) : launch_t<Store::FP16, false>(st, L);
+        if (!tiled_wide_staging(L.g, L.store)) return launch_t<Store::FP16, false>(st, L);
+        return tiled_tile_width(L.g, L.store) == 32 ? launch_t<Store::FP16, true, 32>(st, L)
+                                                    : launch_t<Store::FP16, true>(st, L);
     default: return hipErrorNotSupported;
     }
 }

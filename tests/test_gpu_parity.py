@@ -267,10 +267,12 @@ def test_kernel_is_native(pkg):
 
 @pytest.mark.parametrize("nx,prec,shape", [(128, "fp32", (64, 4)), (96, "fp32", (32, 8)), (32, "fp32", (32, 8)),
                                            (16, "fp32", (32, 8)), (48, "fp32", (64, 4)), (18, "fp32", (64, 2)),
-                                           (96, "bf16", (64, 4)), (96, "fp64", (0, 0))])
+                                           (20, "fp32", (32, 8)), (96, "bf16", (32, 8)), (96, "fp16", (32, 8)),
+                                           (128, "bf16", (64, 4)), (20, "bf16", (64, 2)), (96, "fp64", (0, 0))])
 def test_tile_shape_follows_the_grid(pkg, nx, prec, shape):
-    """FP32 grids whose rows fill 32-column tiles better than 64-column ones take 32 x 8 tiles;
-    rows that are not whole 16-byte granules take 64 x 2; FP64 runs the two-pass kernel."""
+    """Grids whose rows fill 32-column tiles better than 64-column ones take 32 x 8 tiles;
+    rows that are not whole 16-byte granules (4 FP32 or 8 16-bit values) take 64 x 2; FP64
+    runs the two-pass kernel."""
     s = pkg.Solver(pkg.default_params(nx=nx, ny=32, nz=32, precision=prec))
     assert s.tile_shape() == shape
 
@@ -288,7 +290,8 @@ def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
     FP32 storage with nx % 4 == 0 stages whole 16-byte granules (nx = 16, 20, 48 wrap inside
     a tile; ny = 21 leaves a partial row tile); nx = 18 takes the 4-byte staging.  FP32 grids
     with nx = 16, 20, 32 and 96 take 32-column tiles of 8 rows (two rows per wave; ny = 20
-    leaves a partial row tile), the others 64-column tiles.  The
+    leaves a partial row tile), and so do 16-bit grids with nx = 16, 32 and 96; the others
+    take 64-column tiles.  The
     alpha = 0.8 hill has BFL nodes of both branches in the 3/4 and 15/16 families, so the
     interpolated wall branch (q < 1/2) runs on the tiled kernels too."""
     o, s = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED, **shape)

@@ -8,7 +8,8 @@ over the whole device-resident domain.  Workload: the production periodic hill,
 128 spanwise x 256 streamwise x 128 wall-normal nodes (4,194,304 nodes), Smagorinsky
 Cs 0.1, seeded perturbed equilibrium start; synthetic state, no I/O.  Prints one JSON
 line with MLUPS, the HBM roofline of the stream-collide kernel (HIP events on its own
-stream) and the FP64 CPU oracle timed on a bounded sample of the same workload.
+stream), the same step with BF16/FP16 population storage, the serving workload, the FP32
+error against FP64 and the FP64 CPU oracle timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -177,6 +178,55 @@ def event_ms(hip, ev) -> float:
     return float(ms.value)
 
 
+def timed_steps(pkg, prec: str, kern: int, physics: dict, steps: int, warmup: int) -> dict:
+    """`steps` steps of one solve at the bench shape after `warmup` untimed ones, replayed from
+    HIP graphs as the serving loop and a production time loop run them (one launch per step
+    for the tiled kernel, whose flux probe is fused), then 200 eager steps with HIP events
+    around each stream-collide launch."""
+    s = pkg.Solver(pkg.default_params(**GRID, **physics, precision=prec, kernel=kern))
+    # the reference's start (InitialUsingDftFunc): fluid at rest, driven by the body force;
+    # a 1e-5 seeded velocity perturbation breaks the exact symmetry
+    s.init_equilibrium(u0=(0.0, 0.0, 0.0), amp=1e-5, seed=20261203)
+    n = GRID["nx"] * GRID["ny"] * GRID["nz"]
+    chunk = max(2, min(50, steps) // 2 * 2)
+    s.step(warmup % 2)
+    for _ in range(warmup // chunk):
+        s.step_graph(chunk, sync=False)
+    s.step(warmup - warmup % 2 - (warmup // chunk) * chunk, sync=False)
+    s.sync()
+    hip, st, ev = stream_events(pkg, s)
+    t0 = time.perf_counter()
+    assert hip.hipEventRecord(ev[0], st) == 0
+    for _ in range(steps // chunk):
+        s.step_graph(chunk, sync=False)
+    s.step(steps % chunk, sync=False)
+    assert hip.hipEventRecord(ev[1], st) == 0
+    s.sync()
+    wall = time.perf_counter() - t0
+    region_ms = event_ms(hip, ev)
+    # HIP events recorded around each stream-collide launch on the solver's stream, over
+    # eagerly launched steps (each pair also times its own event packets)
+    nev = max(20, min(200, steps))
+    total_ms, collide_ms = s.time_steps(nev)
+    s.sync()
+    finite = bool(all(np.isfinite(a).all() for a in s.macros()))
+    ktag = {pkg.KERNEL_TILED: "tiled", pkg.KERNEL_TWOPASS: "twopass"}.get(s.kernel_in_use(), "pernode")
+    out = {"n": n, "chunk": chunk, "mlups": n * steps / wall / 1e6, "ms_per_step": wall * 1e3 / steps,
+           "finite": finite, "kernel": ktag, "eager_ms_per_step": total_ms / nev,
+           "eager_launch_us": collide_ms * 1e3 / nev}
+    if ktag == "tiled":
+        # one launch per step (no forcing update falls in the region): the kernel's average
+        # launch is the timed region's event time over its steps
+        out["avg_launch_us"] = region_ms * 1e3 / steps
+        out["launch_timing"] = "HIP events around the timed region on the solver's stream, per step (one launch per step)"
+        out["tile"] = "{}x{}".format(*s.tile_shape())
+    else:
+        out["avg_launch_us"] = collide_ms * 1e3 / nev
+        out["launch_timing"] = f"HIP events around each of {nev} eager stream-collide launches"
+    s.close()
+    return out
+
+
 def dropin_timing(pkg, calls: int):
     """Launch_CollisionStreaming (the reference-named entry point, evolution.h:842) on the
     reference's padded FP64 layout at the bench shape: 19 + 19 caller-owned device arrays
@@ -242,6 +292,7 @@ def main() -> int:
     ap.add_argument("--serving-steps", type=int, default=5000)
     ap.add_argument("--serving-compare", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-storage16", action="store_true", help="FP32 runs: skip timing the BF16/FP16 storage forms")
     ap.add_argument("--dropin", type=int, default=0, help="also time N Launch_CollisionStreaming calls (FP64)")
     ap.add_argument("--grid", default="128,256,128", help="NX,NY,NZ (spanwise, streamwise, wall-normal)")
     args = ap.parse_args()
@@ -255,71 +306,40 @@ def main() -> int:
     kern = {"auto": pkg.KERNEL_AUTO, "pernode": pkg.KERNEL_PERNODE, "tiled": pkg.KERNEL_TILED,
             "twopass": pkg.KERNEL_TWOPASS}[args.kernel]
     phys = PHYSICS[args.physics]
-    p = pkg.default_params(**GRID, **phys, precision=args.precision, kernel=kern)
-    s = pkg.Solver(p)
-    # the reference's start (InitialUsingDftFunc): fluid at rest, driven by the body force;
-    # a 1e-5 seeded velocity perturbation breaks the exact symmetry
-    s.init_equilibrium(u0=(0.0, 0.0, 0.0), amp=1e-5, seed=20261203)
-    n = GRID["nx"] * GRID["ny"] * GRID["nz"]
-    print(f"[bench] {args.precision} {args.physics} kernel={s.kernel_in_use()} warmup {args.warmup}",
-          file=sys.stderr, flush=True)
-    # Steps run as the serving loop and a production time loop run them: chunks replayed
-    # from a HIP graph (one launch per step for the tiled kernel, whose flux probe is fused).
-    chunk = max(2, min(50, args.steps) // 2 * 2)
-    s.step(args.warmup % 2)
-    for _ in range(args.warmup // chunk):
-        s.step_graph(chunk, sync=False)
-    s.step(args.warmup - args.warmup % 2 - (args.warmup // chunk) * chunk, sync=False)
-    s.sync()
-    hip, st, ev = stream_events(pkg, s)
-    t0 = time.perf_counter()
-    assert hip.hipEventRecord(ev[0], st) == 0
-    for _ in range(args.steps // chunk):
-        s.step_graph(chunk, sync=False)
-    s.step(args.steps % chunk, sync=False)
-    assert hip.hipEventRecord(ev[1], st) == 0
-    s.sync()
-    wall = time.perf_counter() - t0
-    region_ms = event_ms(hip, ev)
-    # HIP events recorded around each stream-collide launch on the solver's stream, over
-    # eagerly launched steps (each pair also times its own event packets)
-    nev = max(20, min(200, args.steps))
-    total_ms, collide_ms = s.time_steps(nev)
-    s.sync()
-    mac = s.macros()
-    finite = bool(all(np.isfinite(a).all() for a in mac))
-    value = n * args.steps / wall / 1e6
-    ktag = {pkg.KERNEL_TILED: "tiled", pkg.KERNEL_TWOPASS: "twopass"}.get(s.kernel_in_use(), "pernode")
-    if ktag == "tiled":
-        # one launch per step (the flux probe is fused; no forcing update falls in the region):
-        # the kernel's average launch is the timed region's event time over its steps
-        avg_collide_s = region_ms / 1e3 / args.steps
-        launch_timing = "HIP events around the timed region on the solver's stream, per step (one launch per step)"
-    else:
-        avg_collide_s = collide_ms / 1e3 / nev
-        launch_timing = f"HIP events around each of {nev} eager stream-collide launches"
+    print(f"[bench] {args.precision} {args.physics} warmup {args.warmup}", file=sys.stderr, flush=True)
+    t = timed_steps(pkg, args.precision, kern, phys, args.steps, args.warmup)
+    ktag = t["kernel"]
     bpl = BYTES_PER_LUP[args.precision]
-    achieved = bpl * n / avg_collide_s / 1e9
-    traffic = traffic_from_profiles(ktag, args.precision)
     out = {
-        "metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+        "metric": METRIC, "value": round(t["mlups"], 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t["ms_per_step"], 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
                                                           "fp16": "f16-storage/f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"periodic hill ISLBM D3Q19 MRT, {grid_tag()} (spanwise x streamwise x wall-normal)",
                    "precision": args.precision, "physics": args.physics, **phys, "kernel": ktag,
-                   "state_finite": finite, "timed_as": f"HIP graph chunks of {chunk} steps",
-                   "eager_event_ms_per_step": round(total_ms / nev, 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                   "state_finite": t["finite"], "timed_as": f"HIP graph chunks of {t['chunk']} steps",
+                   "eager_event_ms_per_step": round(t["eager_ms_per_step"], 4)},
+        "roofline": {"bound": "hbm", "achieved": round(bpl * t["n"] / t["avg_launch_us"] / 1e3, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bpl * t["n"] / t["avg_launch_us"] / 1e3 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(ktag, args.precision),
                      "kernel": kernel_name(ktag, args.precision), "bytes_per_lup": bpl,
-                     "avg_launch_us": round(avg_collide_s * 1e6, 2), "launch_timing": launch_timing,
-                     "eager_event_launch_us": round(collide_ms * 1e3 / nev, 2)},
+                     "avg_launch_us": round(t["avg_launch_us"], 2), "launch_timing": t["launch_timing"],
+                     "eager_event_launch_us": round(t["eager_launch_us"], 2)},
     }
     if ktag == "tiled":
-        out["roofline"]["tile"] = "{}x{}".format(*s.tile_shape())  # columns x rows of a workgroup's tile
-    s.close()
+        out["roofline"]["tile"] = t["tile"]  # columns x rows of a workgroup's tile
+    if args.precision == "fp32" and not args.no_storage16:
+        # the same step with populations stored as 16-bit scaled deviations, timed the same way
+        out["storage_16bit"] = {}
+        for prec in ("bf16", "fp16"):
+            u = timed_steps(pkg, prec, kern, phys, args.steps, args.warmup)
+            out["storage_16bit"][prec] = {
+                "avg_launch_us": round(u["avg_launch_us"], 2), "ratio_vs_fp32": round(u["avg_launch_us"] / t["avg_launch_us"], 3),
+                "mlups": round(u["mlups"], 1), "tile": u.get("tile"), "bytes_per_lup": BYTES_PER_LUP[prec],
+                "frac": round(BYTES_PER_LUP[prec] * u["n"] / u["avg_launch_us"] / 1e3 / HBM_PEAK_GBS, 4),
+                "state_finite": u["finite"]}
     try:
         out["roofline"]["copy_gbs_measured"] = round(pkg.copy_bandwidth(1 << 30, 20), 1)
     except Exception as e:  # diagnostic only

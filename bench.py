@@ -293,7 +293,8 @@ def main() -> int:
     ap.add_argument("--serving-compare", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-storage16", action="store_true", help="FP32 runs: skip timing the BF16/FP16 storage forms")
-    ap.add_argument("--dropin", type=int, default=0, help="also time N Launch_CollisionStreaming calls (FP64)")
+    ap.add_argument("--dropin", type=int, default=20,
+                    help="time N Launch_CollisionStreaming calls on the reference's padded FP64 layout (0: skip)")
     ap.add_argument("--grid", default="128,256,128", help="NX,NY,NZ (spanwise, streamwise, wall-normal)")
     args = ap.parse_args()
     GRID.update(zip(("nx", "ny", "nz"), (int(v) for v in args.grid.split(","))))

@@ -308,30 +308,34 @@ def test_tiled_kernel_against_fp64_oracle(pkg, po, prec, tol, shape):
 
 
 # the bench's physics (Re_h 2800, U_b 0.05, Smagorinsky Cs 0.1, tanh stretch 2.0, clamped
-# near-wall xi) on a 64 x 128 x 64 grid: 64-column tiles, four z chunks of 16 levels
-PRODUCTION = dict(nx=64, ny=128, nz=64, tau=0.0, re=2800.0, u_ref=0.05, cs_smag=0.1, mesh_mode=1, stretch=2.0,
-                  xi_near_wall=1)
+# near-wall xi) on 64 x 128 x 64 (64-column tiles, four z chunks of 16 levels) and on
+# 96 x 128 x 48 (32-column tiles in every storage, three chunks)
+PRODUCTION = dict(tau=0.0, re=2800.0, u_ref=0.05, cs_smag=0.1, mesh_mode=1, stretch=2.0, xi_near_wall=1)
 PRODUCTION_STEPS = 100
 
 
-@pytest.fixture(scope="module")
-def production_oracle(po):
-    o = po.Oracle(po.default_params(**dict(ndtfrc=10000, **PRODUCTION)))
+@pytest.fixture(scope="module", params=[((64, 128, 64), (64, 4)), ((96, 128, 48), (32, 8))],
+                ids=["64x128x64", "96x128x48"])
+def production_oracle(request, po):
+    (nx, ny, nz), tile = request.param
+    kw = dict(nx=nx, ny=ny, nz=nz, ndtfrc=10000, **PRODUCTION)
+    o = po.Oracle(po.default_params(**kw))
     o.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=11)
     o.step(PRODUCTION_STEPS)
-    return o
+    return o, kw, tile
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
 def test_tiled_kernel_at_the_production_point(pkg, production_oracle, prec, tol):
     """The tiled kernels at the bench's physics (LES at Re_h 2800, stretched mesh, clamped
     near-wall xi weights) against the FP64 oracle after 100 steps from the same seeded field
-    (measured rel L2 of u: FP32 1.5e-7, FP16 3.7e-4, BF16 4.9e-3)."""
-    s = pkg.Solver(pkg.default_params(precision=prec, kernel=pkg.KERNEL_TILED, ndtfrc=10000, **PRODUCTION))
-    assert s.tile_shape() == (64, 4)
+    (measured rel L2 of u at 64 x 128 x 64: FP32 1.5e-7, FP16 3.7e-4, BF16 4.9e-3)."""
+    o, kw, tile = production_oracle
+    s = pkg.Solver(pkg.default_params(precision=prec, kernel=pkg.KERNEL_TILED, **kw))
+    assert s.tile_shape() == tile
     s.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=11)
     s.step(PRODUCTION_STEPS)
-    compare(production_oracle, s, tol, f"tiled {prec} production point {PRODUCTION_STEPS}")
+    compare(o, s, tol, f"tiled {prec} production point {kw['nx']}x{kw['ny']}x{kw['nz']} {PRODUCTION_STEPS}")
 
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])

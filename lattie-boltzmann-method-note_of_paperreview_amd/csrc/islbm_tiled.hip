@@ -2,8 +2,8 @@
 // FP32 and 16-bit population storage (FP32 arithmetic, deviation form f - w_q).
 //
 // Work decomposition.  A workgroup owns a 64 (x) by TY (y) tile of node columns (TY = 4 with
-// 16-byte staging, 2 with 4-byte staging; FP32 grids that fill 32-column tiles better take
-// 32 x 8 tiles, two rows per wave) and marches up a chunk of z levels.  It has 3 x 4 (or
+// 16-byte staging, 2 with 4-byte staging; grids with 16-byte staging that fill 32-column tiles
+// better take 32 x 8 tiles, two rows per wave) and marches up a chunk of z levels.  It has 3 x 4 (or
 // 3 x 2) waves in three roles, one wave of each role per y row (per row pair):
 //   ring waves, group A and group B: a lane keeps a ring of its group's 8 directions x 7
 //       levels in registers
@@ -50,12 +50,13 @@ namespace {
 // l % TX of row R (w % NWR) + l / TX.  Waves w, w + 4, w + 8 share a SIMD.
 //   W = false (4-byte staging, TX = 64, TY = 2): rows with an x halo are 70 wide from
 //       x = i0 - 3.
-//   W = true (16-byte staging, TY = 4 R): rows with an x halo are TX + 8 wide from x = i0 - 4
-//       (whole 16-byte granules); each SIMD then holds one wave of each role.  A level is
+//   W = true (16-byte staging, TY = 4 R): rows with an x halo are TX + 2 GV wide from
+//       x = i0 - GV (whole 16-byte granules of GV = 4 FP32 or 8 16-bit values); each SIMD
+//       then holds one wave of each role.  A level is
 //       staged by NW16 dwordx4 LDS-DMA loads of 64 lanes, lane l of load n filling granule
 //       64 n + l (16 bytes at float 4 (64 n + l)).  TX = 64 (one row per wave), or TX = 32
-//       (FP32, two rows per wave) for grids whose rows fill 32-column tiles but not 64-column
-//       ones (nx = 32, 96, ...), where a 64-column tile would leave lanes idle.
+//       (two rows per wave) for grids whose rows fill 32-column tiles but not 64-column ones
+//       (nx = 32, 96, ...), where a 64-column tile would leave lanes idle.
 // A staged level holds
 //   XY: 7-10 [4][HY][HXW]   X: 11-14 [4][TY][HXW]   Z: 5, 6 [2][TY][TX]   Y: 3 4 15-18 [6][HY][TX]
 // (16-byte staging: XY, X, Z, Y in that order, contiguous, padded to NW16 granule rows).

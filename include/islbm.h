@@ -63,6 +63,10 @@ extern "C" {
                                  /* (16 doubles per node, counted by islbm_required_bytes), */
                                  /* then the xi sums, then walls + collision in place;      */
                                  /* ISLBM_ERR_UNSUPPORTED for the other precisions          */
+#define ISLBM_KERNEL_TILED_WIDE 4 /* the tiled kernel reading each population array through */
+                                  /* its own 64-bit base (wide addressing): chosen by AUTO and */
+                                  /* TILED for grids whose 19 arrays exceed 4 GiB; selectable */
+                                  /* for any grid the tiled kernel takes                      */
 
 /* Run-time form of the reference's compile-time parameter set (variables.h:5-102). */
 typedef struct islbm_params {
@@ -200,7 +204,8 @@ const char* islbm_io_last_error(void);
 int islbm_scalars(islbm_solver* s, double out[8]);
 /* Steps taken since creation / the last init. */
 int islbm_step_count(islbm_solver* s, int64_t* steps);
-/* Kernel in use (ISLBM_KERNEL_PERNODE, ISLBM_KERNEL_TILED or ISLBM_KERNEL_TWOPASS). */
+/* Kernel in use (ISLBM_KERNEL_PERNODE, ISLBM_KERNEL_TILED, ISLBM_KERNEL_TILED_WIDE or
+ * ISLBM_KERNEL_TWOPASS). */
 int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel);
 /* Columns x rows of node columns one workgroup of the tiled kernel owns: 64 x 4, 32 x 8
  * (grids whose rows fill 32-column tiles better than 64-column ones) or 64 x 2 (rows that

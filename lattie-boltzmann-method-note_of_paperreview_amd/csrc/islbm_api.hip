@@ -131,6 +131,7 @@ struct islbm_solver {
     Store store = Store::F64;
     bool fp64 = true;
     int kernel = ISLBM_KERNEL_PERNODE;
+    bool wide_addr = false;  // tiled kernel with one resource per population array
     hipStream_t stream = nullptr;
     long long n = 0;
     long long qs = 0;  // population stride (pop_stride)
@@ -356,6 +357,7 @@ StepLaunch make_launch(islbm_solver* s, int cur)
     L.flux_plane = s->g.flux_plane;
     L.flux_par = cur;
     L.flux_nslot = s->flux_nslot;
+    L.wide_addr = s->wide_addr;
     return L;
 }
 
@@ -439,7 +441,7 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
     if (!pin || !out) return fail(ISLBM_ERR_ARG, "null argument");
     *out = nullptr;
     if (pin->precision < ISLBM_FP64 || pin->precision > ISLBM_FP16) return fail(ISLBM_ERR_ARG, "unknown precision");
-    if (pin->kernel < ISLBM_KERNEL_AUTO || pin->kernel > ISLBM_KERNEL_TWOPASS) return fail(ISLBM_ERR_ARG, "unknown kernel");
+    if (pin->kernel < ISLBM_KERNEL_AUTO || pin->kernel > ISLBM_KERNEL_TILED_WIDE) return fail(ISLBM_ERR_ARG, "unknown kernel");
     islbm_solver* s = new islbm_solver();
     s->p = *pin;
     std::string msg = build_host_geometry(s->p, s->g);
@@ -462,7 +464,10 @@ int islbm_create(const islbm_params* pin, size_t budget, islbm_solver** out)
         s->kernel = s->fp64 ? ISLBM_KERNEL_TWOPASS
                             : (tiled_supported(lg, s->store, s->fp64) ? ISLBM_KERNEL_TILED : ISLBM_KERNEL_PERNODE);
     else
-        s->kernel = s->p.kernel;
+        s->kernel = s->p.kernel == ISLBM_KERNEL_TILED_WIDE ? ISLBM_KERNEL_TILED : s->p.kernel;
+    // wide addressing: asked for, or needed by the grid
+    s->wide_addr = s->kernel == ISLBM_KERNEL_TILED &&
+                   (s->p.kernel == ISLBM_KERNEL_TILED_WIDE || tiled_needs_wide(lg, s->store));
     if (s->kernel == ISLBM_KERNEL_TILED && !tiled_supported(lg, s->store, s->fp64)) {
         delete s;
         return fail(ISLBM_ERR_UNSUPPORTED, "tiled kernel does not support this grid/precision");
@@ -916,7 +921,7 @@ int islbm_step_count(islbm_solver* s, int64_t* steps)
 int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel)
 {
     if (!s || !kernel) return fail(ISLBM_ERR_ARG, "null argument");
-    *kernel = s->kernel;
+    *kernel = s->kernel == ISLBM_KERNEL_TILED && s->wide_addr ? ISLBM_KERNEL_TILED_WIDE : s->kernel;
     return ISLBM_OK;
 }
 
@@ -1073,8 +1078,8 @@ int islbm_copy_bandwidth(int32_t device, size_t bytes, int32_t iters, double* gb
 bool islbm::group_eligible(islbm_solver* s)
 {
     const LaunchGeom lg{s->p.nx, s->p.ny, s->p.nz, s->n, s->qs};
-    return s->kernel == ISLBM_KERNEL_TILED && tiled_groupable(lg, s->store) && !(s->st_acc && s->st_every > 0) &&
-           s->auto_force;
+    return s->kernel == ISLBM_KERNEL_TILED && !s->wide_addr && tiled_groupable(lg, s->store) &&
+           !(s->st_acc && s->st_every > 0) && s->auto_force;
 }
 
 int islbm::group_limit(islbm_solver* s, int64_t want)

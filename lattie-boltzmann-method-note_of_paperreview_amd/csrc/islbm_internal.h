@@ -97,6 +97,7 @@ struct StepLaunch {
     int flux_plane;       // unique j of the probe plane
     int flux_par;         // parity of this step: writes flux_part[par], adds flux_part[1 - par]
     int flux_nslot;
+    bool wide_addr;       // tiled kernel: one resource per population array (forced, or for grids that need it)
 };
 hipError_t launch_stream_collide(hipStream_t st, const StepLaunch& L);
 hipError_t launch_stream_copy(hipStream_t st, const void* src, void* dst, long long bytes);
@@ -109,6 +110,9 @@ hipError_t launch_init_equilibrium(hipStream_t st, Store store, void* f, long lo
                                    int ny, int nz, double rho0, const double u0[3], double amp, uint64_t seed,
                                    const float* scale);
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64);
+// The grid's 19 population arrays (or rho, u, v, w) exceed one buffer resource's 4 GiB: the
+// tiled kernel then reads each array through a resource of its own (wide addressing).
+bool tiled_needs_wide(const LaunchGeom& g, Store store);
 // Grouped stepping (serving): one launch advances up to 8 FP32 solves whose grids take the
 // tiled kernel with 16-byte staging (tiled_groupable) by one step each; solve e takes part
 // while step < limit[e].  Results equal those of launch_stream_collide for each solve.

@@ -297,6 +297,51 @@ __device__ inline unsigned granule_offset(int G, int i0, int j0, int nx, int ny,
     return (unsigned)q * nbq + (unsigned)VB * (unsigned)(y * nx + x);
 }
 
+// Wide addressing (each population array through a resource of its own, from its 64-bit
+// base): the 16-byte staging loads of a level, each within the granules of one direction,
+// so that a load takes the resource of its array.  The 16 staged directions in region order
+// (XY 7-10, X 11-14, Z 5 6, Y 3 4 15-18); load n covers granules [g0, g0 + cnt) of
+// direction q, at the same places in LDS as with one resource for all arrays.
+template <int VB, int TXW>
+struct WideLoads {
+    using LW = Lay<true, VB, TXW>;
+    __host__ __device__ static constexpr int dir_q(int d)
+    {
+        return d < 4 ? 7 + d : d < 8 ? 7 + d : d < 10 ? d - 3 : qy(d - 10);
+    }
+    __host__ __device__ static constexpr int dir_g0(int d)
+    {
+        return d < 4    ? (LW::OFF_XY + d * LW::XY_SZ) / LW::GV
+               : d < 8  ? (LW::OFF_X + (d - 4) * LW::X_SZ) / LW::GV
+               : d < 10 ? (LW::OFF_Z + (d - 8) * LW::TY * LW::TX) / LW::GV
+                        : (LW::OFF_Y + (d - 10) * LW::Y_SZ) / LW::GV;
+    }
+    __host__ __device__ static constexpr int dir_n(int d)
+    {
+        return (d < 4 ? LW::XY_SZ : d < 8 ? LW::X_SZ : d < 10 ? LW::TY * LW::TX : LW::Y_SZ) / LW::GV;
+    }
+    __host__ __device__ static constexpr int dir_loads(int d) { return (dir_n(d) + 63) / 64; }
+    __host__ __device__ static constexpr int count()
+    {
+        int c = 0;
+        for (int d = 0; d < 16; ++d) c += dir_loads(d);
+        return c;
+    }
+    static constexpr int N = count();
+    __host__ __device__ static constexpr void of(int n, int& q, int& g0, int& cnt)
+    {
+        int d = 0;
+        while (d < 15 && n >= dir_loads(d)) n -= dir_loads(d++);
+        q = dir_q(d);
+        g0 = dir_g0(d) + 64 * n;
+        cnt = dir_n(d) - 64 * n < 64 ? dir_n(d) - 64 * n : 64;
+    }
+    // the regions are contiguous and end where the staged level ends
+    static_assert(dir_g0(15) + dir_n(15) == LW::USED / LW::GV && dir_g0(4) == dir_g0(3) + dir_n(3) &&
+                      dir_g0(8) == dir_g0(7) + dir_n(7) && dir_g0(10) == dir_g0(9) + dir_n(9),
+                  "staging regions");
+};
+
 // a 16-bit population code as an FP32 value, before its per-direction scale
 template <Store ST>
 __device__ inline float dec16(unsigned short h)
@@ -309,7 +354,7 @@ __device__ inline float dec16(unsigned short h)
 
 // One workgroup's work of the stream-collide step of one solve (bid: its block index
 // within that solve's launch).  Inlined into the single-solve kernel and the grouped one.
-template <Store ST, bool W, int TXW = 64>
+template <Store ST, bool W, int TXW = 64, bool BIG = false>
 __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<ST>& a, const TiledGeom& tg,
                                                                   const int bid)
 {
@@ -354,8 +399,18 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     const int rl = R == 1 ? 0 : lane / TX;
     const int i = i0 + tx;
     const int iw = i < nx ? i : i % nx;
-    const auto rfo = rsrc_of(a.fo, NQ * nbq);
-    auto qoff = [&](int q) { return (unsigned)q * nbq; };
+    // A population buffer is read through one resource for its 19 arrays (the direction's
+    // offset goes into the load's offsets), or with wide addressing (BIG: grids whose 19
+    // arrays exceed 4 GiB) through one resource per array, from its 64-bit base.
+    const unsigned long long nbq64 = (unsigned long long)SZ * (unsigned long long)a.qs;
+    const auto rfo = rsrc_of(a.fo, BIG ? nbq : NQ * nbq);
+    auto qoff = [&](int q) { return BIG ? 0u : (unsigned)q * nbq; };
+    auto src = [&](int q) {
+        if constexpr (BIG)
+            return rsrc_of((const char*)a.fo + (unsigned long long)q * nbq64, nbq);
+        else
+            return rfo;
+    };
     // directions 0, 1, 2 of a target (no z dependence) from its staged rows P: the node's own
     // 0 and the 7 x taps of 1 and 2 (weights w1, w2), in FP32 deviation form
     auto reduce012 = [&](const float* P, const float* w1, const float* w2, float& f0, float& f1, float& f2) {
@@ -419,12 +474,23 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         unsigned yrow[HY];  // byte offset of staged row rr in a plane
 #pragma unroll
         for (int rr = 0; rr < HY; ++rr) yrow[rr] = SZ * (unsigned)(wrap1(j0 - 3 + rr, ny) * nx);
-        // 16-byte staging: this wave issues loads n = w, w + 4, ... (< NW16) of each level
-        unsigned g16[(NW16 + NRW - 1) / NRW];
+        // 16-byte staging: this wave issues loads n = w, w + NRW, ... (< NL) of each level.  With
+        // wide addressing load n covers granules [wg0, wg0 + wcnt) of direction wq (wave-uniform)
+        // and the lane's offset is within that direction's array.
+        using WL = WideLoads<W ? (int)SZ : 4, TX>;
+        constexpr int NL = BIG && W ? WL::N : NW16;
+        unsigned g16[(NL + NRW - 1) / NRW];
+        int wq[(NL + NRW - 1) / NRW], wg0[(NL + NRW - 1) / NRW], wcnt[(NL + NRW - 1) / NRW];
         if constexpr (W) {
 #pragma unroll
-            for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m)
-                g16[m] = granule_offset<W ? (int)SZ : 4, TX>(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
+            for (int m = 0; m < (NL + NRW - 1) / NRW; ++m) {
+                if constexpr (BIG) {
+                    WL::of(min(w + NRW * m, NL - 1), wq[m], wg0[m], wcnt[m]);
+                    g16[m] = granule_offset<W ? (int)SZ : 4, TX>(wg0[m] + lane, i0, j0, nx, ny, 0u);
+                } else {
+                    g16[m] = granule_offset<W ? (int)SZ : 4, TX>(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
+                }
+            }
         }
         float wx[2][7], wy[2][7];
 #pragma unroll
@@ -447,7 +513,15 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         auto fetch = [&](int lev, int b) {
             const unsigned lv = SZ * (unsigned)lev * (unsigned)plane;
             float* L = lds + b * STAGE;  // b < NRAW
-            if constexpr (W) {
+            if constexpr (W && BIG) {
+#pragma unroll
+                for (int m = 0; m < (NL + NRW - 1) / NRW; ++m) {
+                    if (w + NRW * m < NL)
+                        if (lane < wcnt[m])  // a direction's last load leaves lanes idle
+                            lds_dma16(src(wq[m]), L + 4 * wg0[m], g16[m], lv);
+                }
+                return;
+            } else if constexpr (W) {
 #pragma unroll
                 for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m) {
                     const int n = w + NRW * m;
@@ -461,26 +535,26 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
 #pragma unroll
             for (int rr = 0; rr < HY; ++rr) {
                 float* dst = L + OFF_XY + w * XY_SZ + rr * HX;
-                dma(rfo, xh, od + yrow[rr], dst);
-                dma(rfo, xh6, od + yrow[rr], dst + 6);
+                dma(src(7 + w), xh, od + yrow[rr], dst);
+                dma(src(7 + w), xh6, od + yrow[rr], dst + 6);
             }
             const unsigned ox = qoff(11 + w) + lv;
 #pragma unroll
             for (int rr = 0; rr < TY; ++rr) {
                 float* dst = L + OFF_X + w * X_SZ + rr * HX;
-                dma(rfo, xh, ox + yrow[3 + rr], dst);
-                dma(rfo, xh6, ox + yrow[3 + rr], dst + 6);
+                dma(src(11 + w), xh, ox + yrow[3 + rr], dst);
+                dma(src(11 + w), xh6, ox + yrow[3 + rr], dst + 6);
             }
 #pragma unroll
             for (int d = 0; d < 6; ++d) {
                 const unsigned oy = qoff(qy(d)) + lv;
 #pragma unroll
                 for (int r = 0; r < 2; ++r)
-                    dma(rfo, xn, oy + yrow[2 * w + r], L + OFF_Y + d * Y_SZ + (2 * w + r) * TX);
+                    dma(src(qy(d)), xn, oy + yrow[2 * w + r], L + OFF_Y + d * Y_SZ + (2 * w + r) * TX);
             }
             if (w >= TY) {  // group B: 5 and 6 at the nodes of row w - TY, for group A's ring
-                dma(rfo, xn, qoff(5) + lv + yrow[3 + w - TY], L + OFF_Z + (w - TY) * TX);
-                dma(rfo, xn, qoff(6) + lv + yrow[3 + w - TY], L + OFF_Z + (TY + w - TY) * TX);
+                dma(src(5), xn, qoff(5) + lv + yrow[3 + w - TY], L + OFF_Z + (w - TY) * TX);
+                dma(src(6), xn, qoff(6) + lv + yrow[3 + w - TY], L + OFF_Z + (TY + w - TY) * TX);
             }
         };
         // 16-bit storage: decode this wave's share to FP32 deviations in place
@@ -500,10 +574,10 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                 const float* R = lds + b * STAGE;
                 float* D = lds + LY::OFF_DEC + db * LY::DEC_SZ;
 #pragma unroll
-                for (int m = 0; m < (NW16 + NRW - 1) / NRW; ++m) {
+                for (int m = 0; m < (NL + NRW - 1) / NRW; ++m) {
                     const int n = w + NRW * m;
-                    const int G = 64 * n + lane;
-                    if (n < NW16 && G < LY::USED / LY::GV) {
+                    const int G = BIG ? wg0[m] + lane : 64 * n + lane;
+                    if (n < NL && (BIG ? lane < wcnt[m] : G < LY::USED / LY::GV)) {
                         const uint4 c = *(const uint4*)(R + 4 * G);
                         const unsigned cw[4] = {c.x, c.y, c.z, c.w};
                         float v[8];
@@ -551,8 +625,8 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             // its share of the level two above the one it appends (at the top of the column,
             // level L1 again into the free buffer).  Loads complete in order, so a step waits
             // until only the loads of the step before are outstanding (vmcnt = their count).
-            const int nbatch = W ? (NW16 - w + NRW - 1) / NRW : (G == 0 ? 32 : 34);
-            constexpr int nP012 = W ? 1 : 5;
+            const int nbatch = W ? (NL - w + NRW - 1) / NRW : (G == 0 ? 32 : 34);
+            constexpr int nP012 = W ? (BIG ? 3 : 1) : 5;
             constexpr int ntab = G == 1 ? R + nP012 : 0;  // xi of the wave's rows + rows of 0-2, every step
             auto wait_staged = [&](int outstanding) {
                 // the counts a ring wave can leave outstanding (0 waits for everything)
@@ -563,6 +637,9 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                 case 5: __builtin_amdgcn_s_waitcnt(vmcnt_imm(5)); break;
                 case 6: __builtin_amdgcn_s_waitcnt(vmcnt_imm(6)); break;
                 case 7: __builtin_amdgcn_s_waitcnt(vmcnt_imm(7)); break;
+                case 8: __builtin_amdgcn_s_waitcnt(vmcnt_imm(8)); break;
+                case 9: __builtin_amdgcn_s_waitcnt(vmcnt_imm(9)); break;
+                case 10: __builtin_amdgcn_s_waitcnt(vmcnt_imm(10)); break;
                 case 32: __builtin_amdgcn_s_waitcnt(vmcnt_imm(32)); break;
                 case 40: __builtin_amdgcn_s_waitcnt(vmcnt_imm(40)); break;
                 default: __builtin_amdgcn_s_waitcnt(vmcnt_imm(0)); break;
@@ -579,7 +656,13 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             // group B waves (one load of the 56 weights of its row by lanes 0-55)
             const auto rxi = rsrc_of(a.t.xiw, xi_bytes);
             unsigned p16 = 0;  // 16-byte staging of directions 0-2 (group B): this lane's granule
-            if constexpr (G == 1 && W) {
+            unsigned p16q[3] = {0, 0, 0};  // the same with wide addressing: per direction, within its array
+            if constexpr (G == 1 && W && BIG) {
+                constexpr int GV = LY::GV;
+                const int xs[3] = {i0 + GV * tx, i0 - GV + GV * tx, i0 - GV + GV * tx};
+#pragma unroll
+                for (int q = 0; q < 3; ++q) p16q[q] = SZ * (unsigned)(jw * nx + ((xs[q] % nx) + nx) % nx);
+            } else if constexpr (G == 1 && W) {
                 // granules: direction 0 at the nodes, then 1 and 2 with the x halo from i0 - GV;
                 // lanes TX rl .. TX rl + TX - 1 stage row rl of the wave
                 constexpr int GV = LY::GV, n0 = TX / GV, n12 = HXW / GV;
@@ -609,17 +692,26 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                     // remaining lanes stay idle
                     constexpr int n012 = TX / LY::GV + 2 * (HXW / LY::GV);  // granules of the three rows
                     static_assert(4 * TX == P_ROW || R == 1, "lanes of row rl land at row rl's slot");
-                    if (tx < n012)
-                        lds_dma16(rfo, lds + OFF_P012 + (par * TY + R * wr) * P_ROW, p16, SZ * (unsigned)tk * (unsigned)plane);
+                    float* P = lds + OFF_P012 + (par * TY + R * wr) * P_ROW;
+                    const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
+                    if constexpr (BIG) {
+                        // one load per direction, each row's granules where the single load puts them
+                        constexpr int n0 = TX / LY::GV, n12 = HXW / LY::GV;
+                        if (tx < n0) lds_dma16(src(0), P, p16q[0], lv);
+                        if (tx < n12) lds_dma16(src(1), P + 4 * n0, p16q[1], lv);
+                        if (tx < n12) lds_dma16(src(2), P + 4 * (n0 + n12), p16q[2], lv);
+                    } else {
+                        if (tx < n012) lds_dma16(rfo, P, p16, lv);
+                    }
                 } else if constexpr (G == 1) {
                     const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
                     float* P = lds + OFF_P012 + (par * TY + ty) * P_ROW;
                     const unsigned o = lv + yrow[3 + ty];
-                    dma(rfo, xn, qoff(0) + o, P);
-                    dma(rfo, xh, qoff(1) + o, P + TX);
-                    dma(rfo, xh6, qoff(1) + o, P + TX + 6);
-                    dma(rfo, xh, qoff(2) + o, P + TX + HX);
-                    dma(rfo, xh6, qoff(2) + o, P + TX + HX + 6);
+                    dma(src(0), xn, qoff(0) + o, P);
+                    dma(src(1), xh, qoff(1) + o, P + TX);
+                    dma(src(1), xh6, qoff(1) + o, P + TX + 6);
+                    dma(src(2), xh, qoff(2) + o, P + TX + HX);
+                    dma(src(2), xh6, qoff(2) + o, P + TX + HX + 6);
                 }
             };
             // hand-over targets of the next two steps (k0 where a step hands nothing over: the
@@ -863,9 +955,23 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         const float rho_mod = (float)*(cdouble*)a.rho_modify;
         const float force = (float)*(cdouble*)a.force;
         const unsigned fbytes = 4u * (unsigned)N;
-        const auto rfn = rsrc_of(a.fn, NQ * nbq);
+        const auto rfn = rsrc_of(a.fn, BIG ? nbq : NQ * nbq);
+        auto dst = [&](int q) {
+            if constexpr (BIG)
+                return rsrc_of((const char*)a.fn + (unsigned long long)q * nbq64, nbq);
+            else
+                return rfn;
+        };
         // rho, u, v, w are consecutive arrays of one allocation (checked at launch)
-        const auto rmac = rsrc_of(a.rho, 4 * fbytes);
+        const auto rmac = rsrc_of(a.rho, BIG ? fbytes : 4 * fbytes);
+        // rho, u, v, w: one resource, or with wide addressing one per field
+        auto macdst = [&](int f) {
+            if constexpr (BIG)
+                return rsrc_of((const char*)a.rho + (unsigned long long)f * fbytes, fbytes);
+            else
+                return rmac;
+        };
+        auto macoff = [&](int f) { return BIG ? 0u : (unsigned)f * fbytes; };
         // Bulk-flux probe (AccumulateUbulk, evolution.h:826-840) fused into the step.  The
         // reference samples v of the previous step on the probe plane before each step; here
         // the waves of the probe row sum v dx_i dz_k of the values they store, per lane over
@@ -903,7 +1009,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                 // 0.5-0.9 us slower that way and keep the global-pointer reads
                 if constexpr (ST == Store::BF16) {
                     auto pop = [&](int q) {
-                        return [&, q](long long n) { return IO::ld(rfo, SZ * (unsigned)n, qoff(q), a.inv_scale[q]); };
+                        return [&, q](long long n) { return IO::ld(src(q), SZ * (unsigned)n, qoff(q), a.inv_scale[q]); };
                     };
                     apply_walls_with<ST>(F, a, pop, node, iw, jw, tk, xo, yo, plane);
                 } else {
@@ -929,11 +1035,11 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) IO::st(rfn, own_b, qoff(q) + lv, F[q], a.scale[q]);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), rmac, (int)own4, (int)lf, kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), rmac, (int)own4, (int)(lf + fbytes), kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), rmac, (int)own4, (int)(lf + 2 * fbytes), kStoreAux);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), rmac, (int)own4, (int)(lf + 3 * fbytes), kStoreAux);
+            for (int q = 0; q < NQ; ++q) IO::st(dst(q), own_b, qoff(q) + lv, F[q], a.scale[q]);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), macdst(0), (int)own4, (int)(lf + macoff(0)), kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), macdst(1), (int)own4, (int)(lf + macoff(1)), kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), macdst(2), (int)own4, (int)(lf + macoff(2)), kStoreAux);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uz), macdst(3), (int)own4, (int)(lf + macoff(3)), kStoreAux);
         };
         Sched sc{L0 - 1, k0};
         bool adv;
@@ -970,11 +1076,11 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     }
 }
 
-template <Store ST, bool W, int TXW>
+template <Store ST, bool W, int TXW, bool BIG>
 __global__ __launch_bounds__(Lay<W>::NTH) __attribute__((amdgpu_waves_per_eu(3))) void k_collide_tiled(StepArgs<ST> a,
                                                                                                           TiledGeom tg)
 {
-    tiled_body<ST, W, TXW>(a, tg, (int)blockIdx.x);
+    tiled_body<ST, W, TXW, BIG>(a, tg, (int)blockIdx.x);
 }
 
 // Several solves advanced by one launch (one step of each): solve e owns blocks
@@ -1029,11 +1135,18 @@ TiledGeom tiled_geom_of(const LaunchGeom& g, Store store)
 
 bool tiled_supported(const LaunchGeom& g, Store store, bool fp64)
 {
-    // buffer resources address a population buffer (19 arrays), a field or the xi table
-    // with 32-bit byte offsets
+    // buffer resources address a population array, a field or the xi table with 32-bit byte
+    // offsets (with wide addressing: one array per resource)
     const long long sz = store == Store::F32 ? 4 : 2;
-    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && NQ * g.qs * sz < (1LL << 32) &&
-           4 * g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
+    return !fp64 && store != Store::F64 && g.nx >= 8 && g.ny >= 8 && g.nz >= 8 && g.qs * sz < (1LL << 32) &&
+           g.n * 4 < (1LL << 32) && (long long)g.nz * g.ny * NXI * 7 * 4 < (1LL << 32);
+}
+
+bool tiled_needs_wide(const LaunchGeom& g, Store store)
+{
+    // one resource for a buffer's 19 arrays, and one for rho, u, v, w
+    const long long sz = store == Store::F32 ? 4 : 2;
+    return NQ * g.qs * sz >= (1LL << 32) || 4 * g.n * 4 >= (1LL << 32);
 }
 
 void tiled_tile_shape(const LaunchGeom& g, Store store, int* cols, int* rows)
@@ -1053,14 +1166,23 @@ int tiled_flux_slots(const LaunchGeom& g, Store store)
     return t.nchunks * t.ntx;
 }
 
-template <Store ST, bool W, int TXW = 64>
+template <Store ST, bool W, int TXW = 64, bool BIG = false>
 static hipError_t launch_t(hipStream_t st, const StepLaunch& L)
 {
     StepArgs<ST> a = make_step_args<ST>(L);
     using LY = Lay<W, W ? (ST == Store::F32 ? 4 : 2) : 4, TXW>;
     TiledGeom tg = tiled_geom(L.g.nx, L.g.ny, L.g.nz, TXW, LY::TY, LY::RESIDENT);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W, TXW>), dim3(tg.nblocks), dim3(LY::NTH), 0, st, a, tg);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collide_tiled<ST, W, TXW, BIG>), dim3(tg.nblocks), dim3(LY::NTH), 0, st, a,
+                       tg);
     return hipGetLastError();
+}
+
+// the tiled kernel with one resource per population array and per field (wide addressing)
+template <Store ST>
+static hipError_t launch_wide(hipStream_t st, const StepLaunch& L)
+{
+    if (!tiled_wide_staging(L.g, L.store)) return launch_t<ST, false, 64, true>(st, L);
+    return tiled_tile_width(L.g, L.store) == 32 ? launch_t<ST, true, 32, true>(st, L) : launch_t<ST, true, 64, true>(st, L);
 }
 
 bool tiled_wide_staging(const LaunchGeom& g, Store store)
@@ -1072,7 +1194,8 @@ bool tiled_wide_staging(const LaunchGeom& g, Store store)
 
 bool tiled_groupable(const LaunchGeom& g, Store store)
 {
-    return store == Store::F32 && tiled_supported(g, store, false) && tiled_wide_staging(g, store);
+    return store == Store::F32 && tiled_supported(g, store, false) && tiled_wide_staging(g, store) &&
+           !tiled_needs_wide(g, store);
 }
 
 // The members of one tile width go into one launch: a group of mixed widths takes two
@@ -1138,6 +1261,14 @@ hipError_t launch_tiled(hipStream_t st, const StepLaunch& L)
     const size_t fb = (size_t)L.g.n * 4;
     if ((const char*)L.u != r + fb || (const char*)L.v != r + 2 * fb || (const char*)L.w != r + 3 * fb)
         return hipErrorInvalidValue;
+    if (L.wide_addr || tiled_needs_wide(L.g, L.store)) {
+        switch (L.store) {
+        case Store::F32: return launch_wide<Store::F32>(st, L);
+        case Store::BF16: return launch_wide<Store::BF16>(st, L);
+        case Store::FP16: return launch_wide<Store::FP16>(st, L);
+        default: return hipErrorNotSupported;
+        }
+    }
     switch (L.store) {
     case Store::F32: {
         // a solve that a group can take runs the grouped kernel alone, so that its steps are

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libislbm_amd.so")
 
 FP64, FP32, BF16, FP16 = 0, 1, 2, 3
 MESH_BISECT, MESH_STRETCH = 0, 1
-KERNEL_AUTO, KERNEL_PERNODE, KERNEL_TILED, KERNEL_TWOPASS = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_PERNODE, KERNEL_TILED, KERNEL_TWOPASS, KERNEL_TILED_WIDE = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "fp16": FP16}
 
 HEADER_SYMBOLS = [

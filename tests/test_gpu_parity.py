@@ -356,6 +356,57 @@ def test_tiled_kernel_at_its_smallest_grids(pkg, po, prec, tol, shape):
     compare(o, s, tol, f"tiled {prec} {shape}")
 
 
+WIDE_SHAPES = [dict(nx=64, ny=32, nz=32), dict(nx=64, ny=32, nz=48, cs_smag=0.1), dict(nx=32, ny=20, nz=24),
+               dict(nx=18, ny=20, nz=24), dict(nx=16, **ALPHA08)]
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp16", 2e-3), ("bf16", 2e-2)])
+@pytest.mark.parametrize("shape", WIDE_SHAPES)
+def test_tiled_wide_addressing_forced_on_small_grids(pkg, po, prec, tol, shape):
+    """The tiled kernel with one buffer resource per population array and per field (wide
+    addressing, what grids past 4 GiB per population buffer take), forced on small grids of
+    every tile form (64 and 32 columns, 4-byte staging, LES, the alpha = 0.8 hill): against
+    the FP64 oracle, and against the default addressing on the same field."""
+    o, s = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED_WIDE, **shape)
+    assert s.kernel_in_use() == pkg.KERNEL_TILED_WIDE
+    _, t = pair(pkg, po, precision=prec, kernel=pkg.KERNEL_TILED, **shape)
+    assert t.kernel_in_use() == pkg.KERNEL_TILED and t.tile_shape() == s.tile_shape()
+    for x in (o, s, t):
+        x.init_equilibrium(u0=(0.0, 0.0583, 0.0), amp=1e-3, seed=9)
+    for x in (o, s, t):
+        x.step(40)
+    compare(o, s, tol, f"wide {prec} {shape}")
+    same = all(np.array_equal(a, b) for a, b in zip(s.macros(), t.macros()))
+    e = pkg.rel_l2_vec(s.macros()[1:], t.macros()[1:])
+    print(f"wide vs default addressing: bit-identical {same}, rel L2 u {e:.2e}", flush=True)
+    assert e <= tol / 10
+
+
+def test_tiled_wide_addressing_on_a_grid_past_4gib(pkg):
+    """FP32 at 384 x 384 x 384 (56.6M nodes: a population buffer's 19 arrays take 4.2 GiB):
+    the default kernel choice is the tiled kernel with wide addressing, and 4 steps agree
+    with the per-node kernel (64-bit indices) on the same field."""
+    kw = dict(nx=384, ny=384, nz=384, tau=0.6833, re=50.0, mesh_mode=0, u_ref=0.0, precision="fp32")
+    a = pkg.Solver(pkg.default_params(**kw))
+    assert a.kernel_in_use() == pkg.KERNEL_TILED_WIDE
+    a.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=3)
+    a.step(4)
+    ma = a.macros()
+    tot, col = a.time_steps(10)
+    print(f"tiled, wide addressing, 384^3 FP32: {col / 10 * 1e3:.1f} us per launch "
+          f"({168 * 384 ** 3 / (col / 10 * 1e-3) / 8e12 * 100:.1f}% of 8 TB/s)", flush=True)
+    a.close()
+    b = pkg.Solver(pkg.default_params(kernel=pkg.KERNEL_PERNODE, **kw))
+    b.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=3)
+    b.step(4)
+    mb = b.macros()
+    b.close()
+    e_u = pkg.rel_l2_vec(ma[1:], mb[1:])
+    e_r = pkg.rel_l2(ma[0], mb[0])
+    print(f"384^3 FP32 tiled (wide) vs per-node after 4 steps: rel L2 u {e_u:.2e} rho {e_r:.2e}", flush=True)
+    assert np.all(np.isfinite(ma[1])) and e_u <= 1e-6 and e_r <= 1e-6
+
+
 @pytest.mark.parametrize("nx", [40, 39])
 def test_fp64_auto_is_twopass_and_equals_pernode(pkg, nx):
     """FP64 solvers take the two-pass kernel by default; it sums the same taps as the

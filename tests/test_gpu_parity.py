@@ -407,6 +407,25 @@ def test_tiled_wide_addressing_on_a_grid_past_4gib(pkg):
     assert np.all(np.isfinite(ma[1])) and e_u <= 1e-6 and e_r <= 1e-6
 
 
+def test_fp64_twopass_on_a_grid_past_4gib(pkg):
+    """FP64 at 384 x 384 x 384 (a population array alone takes 453 MB, a buffer 8.6 GB, the
+    filtered-direction buffer 7.2 GB): the two-pass kernel agrees with the per-node kernel
+    after 2 steps (64-bit indexing throughout)."""
+    kw = dict(nx=384, ny=384, nz=384, tau=0.6833, re=50.0, mesh_mode=0, u_ref=0.0, precision="fp64")
+    out = []
+    for kern in (pkg.KERNEL_AUTO, pkg.KERNEL_PERNODE):
+        s = pkg.Solver(pkg.default_params(kernel=kern, **kw))
+        assert s.kernel_in_use() == (pkg.KERNEL_TWOPASS if kern == pkg.KERNEL_AUTO else pkg.KERNEL_PERNODE)
+        s.init_equilibrium(u0=(0.0, 0.05, 0.0), amp=1e-3, seed=3)
+        s.step(2)
+        out.append(s.macros())
+        s.close()
+    e_u = pkg.rel_l2_vec(out[0][1:], out[1][1:])
+    e_r = pkg.rel_l2(out[0][0], out[1][0])
+    print(f"384^3 FP64 two-pass vs per-node after 2 steps: rel L2 u {e_u:.2e} rho {e_r:.2e}", flush=True)
+    assert e_u <= 1e-12 and e_r <= 1e-12
+
+
 @pytest.mark.parametrize("nx", [40, 39])
 def test_fp64_auto_is_twopass_and_equals_pernode(pkg, nx):
     """FP64 solvers take the two-pass kernel by default; it sums the same taps as the

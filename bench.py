@@ -68,9 +68,12 @@ def traffic_from_profiles(kernel_tag: str, prec: str):
 
 def kernel_name(ktag: str, prec: str) -> str:
     """The dominant kernel's symbol: FP32 on a grid whose rows are whole 16-byte granules
-    runs the grouped tiled kernel (a group of one solve), 16-bit storage the tiled kernel."""
+    runs the grouped tiled kernel (a group of one solve), 16-bit storage the tiled kernel;
+    grids past 4 GiB per population buffer the tiled kernel with wide addressing."""
     if ktag == "twopass":
         return "k_hfilter_lds+k_xi_stream_lds+k_collide_inplace"
+    if ktag == "tiled-wide":
+        return "k_collide_tiled"
     if ktag == "tiled":
         return "k_collide_tiled_group" if prec == "fp32" and GRID["nx"] % 4 == 0 else "k_collide_tiled"
     return "k_collide_pernode"
@@ -210,11 +213,12 @@ def timed_steps(pkg, prec: str, kern: int, physics: dict, steps: int, warmup: in
     total_ms, collide_ms = s.time_steps(nev)
     s.sync()
     finite = bool(all(np.isfinite(a).all() for a in s.macros()))
-    ktag = {pkg.KERNEL_TILED: "tiled", pkg.KERNEL_TWOPASS: "twopass"}.get(s.kernel_in_use(), "pernode")
+    ktag = {pkg.KERNEL_TILED: "tiled", pkg.KERNEL_TILED_WIDE: "tiled-wide",
+            pkg.KERNEL_TWOPASS: "twopass"}.get(s.kernel_in_use(), "pernode")
     out = {"n": n, "chunk": chunk, "mlups": n * steps / wall / 1e6, "ms_per_step": wall * 1e3 / steps,
            "finite": finite, "kernel": ktag, "eager_ms_per_step": total_ms / nev,
            "eager_launch_us": collide_ms * 1e3 / nev}
-    if ktag == "tiled":
+    if ktag in ("tiled", "tiled-wide"):
         # one launch per step (no forcing update falls in the region): the kernel's average
         # launch is the timed region's event time over its steps
         out["avg_launch_us"] = region_ms * 1e3 / steps
@@ -282,7 +286,7 @@ def main() -> int:
     # long run sustains
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_LUP))
-    ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled", "twopass"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "pernode", "tiled", "tiled-wide", "twopass"])
     ap.add_argument("--physics", default="production-clamped", choices=sorted(PHYSICS))
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -305,6 +309,7 @@ def main() -> int:
 
     pkg = importlib.import_module(PKG)
     kern = {"auto": pkg.KERNEL_AUTO, "pernode": pkg.KERNEL_PERNODE, "tiled": pkg.KERNEL_TILED,
+            "tiled-wide": pkg.KERNEL_TILED_WIDE,
             "twopass": pkg.KERNEL_TWOPASS}[args.kernel]
     phys = PHYSICS[args.physics]
     print(f"[bench] {args.precision} {args.physics} warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -329,7 +334,7 @@ def main() -> int:
                      "avg_launch_us": round(t["avg_launch_us"], 2), "launch_timing": t["launch_timing"],
                      "eager_event_launch_us": round(t["eager_launch_us"], 2)},
     }
-    if ktag == "tiled":
+    if ktag in ("tiled", "tiled-wide"):
         out["roofline"]["tile"] = t["tile"]  # columns x rows of a workgroup's tile
     if args.precision == "fp32" and not args.no_storage16:
         # the same step with populations stored as 16-bit scaled deviations, timed the same way

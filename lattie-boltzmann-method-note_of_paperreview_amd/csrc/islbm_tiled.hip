@@ -405,9 +405,17 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     const unsigned long long nbq64 = (unsigned long long)SZ * (unsigned long long)a.qs;
     const auto rfo = rsrc_of(a.fo, BIG ? nbq : NQ * nbq);
     auto qoff = [&](int q) { return BIG ? 0u : (unsigned)q * nbq; };
+    // a wide-addressing resource is built where it is used: the empty asm keeps the compiler
+    // from computing the 19 of them once and holding them (76 scalar registers, spilled)
+    auto wide_rsrc = [&](const void* base, unsigned long long off, unsigned bytes) {
+        const unsigned long long p = (unsigned long long)base + off;
+        unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)p), hi = __builtin_amdgcn_readfirstlane((unsigned)(p >> 32));
+        asm volatile("" : "+s"(lo), "+s"(hi));
+        return rsrc_of((const void*)(((unsigned long long)hi << 32) | lo), bytes);
+    };
     auto src = [&](int q) {
         if constexpr (BIG)
-            return rsrc_of((const char*)a.fo + (unsigned long long)q * nbq64, nbq);
+            return wide_rsrc(a.fo, (unsigned long long)q * nbq64, nbq);
         else
             return rfo;
     };
@@ -485,7 +493,11 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
 #pragma unroll
             for (int m = 0; m < (NL + NRW - 1) / NRW; ++m) {
                 if constexpr (BIG) {
-                    WL::of(min(w + NRW * m, NL - 1), wq[m], wg0[m], wcnt[m]);
+                    int q_, g0_, cnt_;
+                    WL::of(min(w + NRW * m, NL - 1), q_, g0_, cnt_);
+                    wq[m] = __builtin_amdgcn_readfirstlane(q_);
+                    wg0[m] = __builtin_amdgcn_readfirstlane(g0_);
+                    wcnt[m] = __builtin_amdgcn_readfirstlane(cnt_);
                     g16[m] = granule_offset<W ? (int)SZ : 4, TX>(wg0[m] + lane, i0, j0, nx, ny, 0u);
                 } else {
                     g16[m] = granule_offset<W ? (int)SZ : 4, TX>(64 * (w + NRW * m) + lane, i0, j0, nx, ny, nbq);
@@ -958,7 +970,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         const auto rfn = rsrc_of(a.fn, BIG ? nbq : NQ * nbq);
         auto dst = [&](int q) {
             if constexpr (BIG)
-                return rsrc_of((const char*)a.fn + (unsigned long long)q * nbq64, nbq);
+                return wide_rsrc(a.fn, (unsigned long long)q * nbq64, nbq);
             else
                 return rfn;
         };
@@ -967,7 +979,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         // rho, u, v, w: one resource, or with wide addressing one per field
         auto macdst = [&](int f) {
             if constexpr (BIG)
-                return rsrc_of((const char*)a.rho + (unsigned long long)f * fbytes, fbytes);
+                return wide_rsrc(a.rho, (unsigned long long)f * fbytes, fbytes);
             else
                 return rmac;
         };

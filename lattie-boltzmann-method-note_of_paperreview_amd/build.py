@@ -53,7 +53,29 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_driver(verbose)
     return LIB
+
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+DRIVER = os.path.join(LIB_DIR, "islbm_run")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C time-loop driver (csrc/islbm_run.c): plain C, host compiler, linked against the
+    library and the HIP runtime, found next to it at run time."""
+    tmp = DRIVER + ".tmp"
+    cmd = [os.environ.get("CC", "gcc"), "-O2", "-std=c11", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROCM, "include"), os.path.join(CSRC, "islbm_run.c"), "-o", tmp,
+           "-L", LIB_DIR, "-lislbm_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 or r.stderr.strip():
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

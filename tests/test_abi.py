@@ -103,3 +103,18 @@ def test_batch_rejects_bad_arguments_before_touching_the_gpu(pkg, kw):
     args.update(kw)
     with pytest.raises(pkg.ISLBMError):
         pkg.Batch(**args)
+
+
+def test_c_driver_rejects_bad_arguments_before_the_gpu(pkg):
+    """The C time-loop driver is built beside the library and refuses odd step counts and
+    cadences (two steps per loop iteration) and drop-in runs the reference's calls cannot
+    express, with exit status 2, before it touches a GPU."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(pkg.islbm.LIB_PATH), "islbm_run")
+    assert os.access(exe, os.X_OK), exe
+    for args in (["--steps", "3"], ["--ndtfrc", "7"], ["--ndtmit", "5"], ["--mass", "3"], ["--tbstart", "1"],
+                 ["--dropin", "--precision", "fp32"], ["--dropin", "--init", "1"], ["--dropin", "--amp", "1e-3"],
+                 ["--grid", "8,8"], ["--precision", "fp8"], ["--bogus", "1"], ["--steps"]):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=30)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)

@@ -98,26 +98,29 @@ def cpu_baseline(physics: dict, steps: int):
             "sample": f"FP64 oracle, {steps} steps of the same {grid_tag()} workload ({dt:.1f} s)"}
 
 
-def accuracy_vs_fp64(pkg, prec: str, physics: dict, steps: int):
-    """Relative L2 error of rho and (u, v, w) of the measured precision against the FP64
+def accuracy_vs_fp64(pkg, precs, physics: dict, steps: int):
+    """Relative L2 error of rho and (u, v, w) of each precision in `precs` against the FP64
     two-pass kernel (equal to the FP64 oracle to 1e-10 on the parity grids) after `steps`
     steps on the same seeded field, u = (0, U_ref, 0) + a 1e-3 perturbation, at the bench
     shape."""
     import numpy as np
 
     out = {}
-    for name in ("fp64", prec):
+    for name in ["fp64"] + list(precs):
         s = pkg.Solver(pkg.default_params(**GRID, **physics, precision=name))
         s.init_equilibrium(u0=(0.0, s.scalars()["uref"], 0.0), amp=1e-3, seed=20261203)
         s.step(steps)
         out[name] = s.macros()
         s.close()
     r0, u0, v0, w0 = out["fp64"]
-    r1, u1, v1, w1 = out[prec]
-    e_u = float(np.sqrt(np.sum((u1 - u0) ** 2 + (v1 - v0) ** 2 + (w1 - w0) ** 2) / np.sum(u0 ** 2 + v0 ** 2 + w0 ** 2)))
-    e_r = float(np.linalg.norm(r1 - r0) / np.linalg.norm(r0))
-    return {"steps": steps, "rel_l2_u": e_u, "rel_l2_rho": e_r, "bound_u": {"fp32": 1e-5, "fp16": 6e-3, "bf16": 2e-2,
-                                                                          "fp64": 1e-10}[prec]}
+    res = {}
+    for prec in precs:
+        r1, u1, v1, w1 = out[prec]
+        e_u = float(np.sqrt(np.sum((u1 - u0) ** 2 + (v1 - v0) ** 2 + (w1 - w0) ** 2) / np.sum(u0 ** 2 + v0 ** 2 + w0 ** 2)))
+        e_r = float(np.linalg.norm(r1 - r0) / np.linalg.norm(r0))
+        res[prec] = {"steps": steps, "rel_l2_u": e_u, "rel_l2_rho": e_r,
+                     "bound_u": {"fp32": 1e-5, "fp16": 6e-3, "bf16": 2e-2, "fp64": 1e-10}[prec]}
+    return res
 
 
 def serving(pkg, nreq: int, nsteps: int, ncmp: int):
@@ -353,7 +356,11 @@ def main() -> int:
     if not args.no_serving:
         out["serving"] = serving(pkg, args.serving_requests, args.serving_steps, args.serving_compare)
     if not args.no_accuracy and args.precision != "fp64":
-        out["accuracy_vs_fp64"] = accuracy_vs_fp64(pkg, args.precision, phys, args.accuracy_steps)
+        extra = sorted(out.get("storage_16bit", {}))
+        acc = accuracy_vs_fp64(pkg, [args.precision] + extra, phys, args.accuracy_steps)
+        out["accuracy_vs_fp64"] = acc[args.precision]
+        for prec in extra:
+            out["storage_16bit"][prec]["accuracy_vs_fp64"] = acc[prec]
     if args.dropin > 0:
         out["dropin"] = dropin_timing(pkg, args.dropin)
     if not args.no_cpu_baseline:

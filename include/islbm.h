@@ -37,7 +37,8 @@ extern "C" {
 /* population storage / arithmetic */
 #define ISLBM_FP64 0 /* FP64 storage and arithmetic (the reference's precision)        */
 #define ISLBM_FP32 1 /* FP32 storage and arithmetic                                    */
-#define ISLBM_BF16 2 /* BF16 storage of (f_i - w_i) * s_i, FP32 arithmetic              */
+#define ISLBM_BF16 2 /* BF16 storage of (f_i - w_i) * s_i, FP32 arithmetic; the step   */
+                     /* rounds its stores stochastically (dither: node, step count)   */
 #define ISLBM_FP16 3 /* FP16 storage of (f_i - w_i) * s_i, FP32 arithmetic              */
 
 /* wall-normal mesh */
@@ -204,6 +205,11 @@ const char* islbm_io_last_error(void);
 int islbm_scalars(islbm_solver* s, double out[8]);
 /* Steps taken since creation / the last init. */
 int islbm_step_count(islbm_solver* s, int64_t* steps);
+/* Sets the step count (for a restart: the reference's checkpoint files carry none).  The
+ * count places the forcing-controller updates and, with BF16 storage, dithers the
+ * stochastic rounding of the step's stores, so a restart given the count it stopped at
+ * continues bit for bit.  No counterpart in the reference (its main keeps the count). */
+int islbm_set_step_count(islbm_solver* s, int64_t steps);
 /* Kernel in use (ISLBM_KERNEL_PERNODE, ISLBM_KERNEL_TILED, ISLBM_KERNEL_TILED_WIDE or
  * ISLBM_KERNEL_TWOPASS). */
 int islbm_kernel_in_use(islbm_solver* s, int32_t* kernel);

@@ -7,7 +7,8 @@
  *   set-up   AllocateMemory, GenerateMesh_X/Y/Z, GetIntrplParameter_X/Y/Xi, BFLInitialization
  *            (memory.h:51-171, initialization.h:51-360)               -> islbm_create
  *   start    INIT == 0: InitialUsingDftFunc (initialization.h:6-49)    -> islbm_init_equilibrium
- *            INIT == 1: InitialUsingBkpData (fileIO.h:314-354)         -> islbm_checkpoint_read
+ *            INIT == 1: InitialUsingBkpData (fileIO.h:314-354)         -> islbm_checkpoint_read,
+ *            with the step count it resumes from (--step0)             -> islbm_set_step_count
  *   loop     two steps per iteration, Launch_CollisionStreaming(ft, fd) then (fd, ft)
  *            (evolution.h:842); TBSWITCH: Launch_TurbulentSum after every step from step
  *            `tbstart` on (statistics.h:224); every NDTFRC steps Launch_ModifyForcingTerm
@@ -54,6 +55,7 @@ static double weight(int q) { return q == 0 ? 1.0 / 3.0 : q <= 6 ? 1.0 / 18.0 : 
 typedef struct {
     islbm_params p;
     int steps, ndtmit, tbstart, mass, init, dropin;
+    long long step0;
     double u0[3], amp;
     uint64_t seed;
     const char* dir;
@@ -85,7 +87,8 @@ static void usage(void)
             "usage: islbm_run [--grid NX,NY,NZ] [--steps N] [--precision fp64|fp32|bf16|fp16]\n"
             "                 [--tau T] [--re R] [--uref U] [--cs C] [--alpha A] [--stretch S]\n"
             "                 [--ndtfrc N] [--ndtmit N] [--tbstart STEP] [--mass N] [--xi-clamped]\n"
-            "                 [--u0 UX,UY,UZ] [--amp A] [--seed S] [--init 0|1] [--dir DIR] [--dropin]\n");
+            "                 [--u0 UX,UY,UZ] [--amp A] [--seed S] [--init 0|1] [--step0 N] [--dir DIR]\n"
+            "                 [--dropin]\n");
 }
 
 static int parse(int argc, char** argv, options* o)
@@ -96,6 +99,7 @@ static int parse(int argc, char** argv, options* o)
     o->tbstart = -1;
     o->mass = 0;
     o->init = 0;
+    o->step0 = 0;
     o->dropin = 0;
     o->u0[0] = 0.0;
     o->u0[1] = 0.0;
@@ -155,6 +159,8 @@ static int parse(int argc, char** argv, options* o)
             o->seed = strtoull(v, NULL, 10);
         } else if (!strcmp(k, "--init")) {
             o->init = atoi(v);
+        } else if (!strcmp(k, "--step0")) {
+            o->step0 = atoll(v);
         } else if (!strcmp(k, "--dir")) {
             o->dir = v;
         } else {
@@ -164,12 +170,12 @@ static int parse(int argc, char** argv, options* o)
         a += used;
     }
     /* two steps per loop iteration; the cadences count in those steps */
-    if (o->steps < 0 || o->steps % 2 || o->p.ndtfrc <= 0 || o->p.ndtfrc % 2 || o->ndtmit < 0 || o->ndtmit % 2 ||
+    if (o->steps < 0 || o->steps % 2 || o->step0 < 0 || o->p.ndtfrc <= 0 || o->p.ndtfrc % 2 || o->ndtmit < 0 || o->ndtmit % 2 ||
         o->mass < 0 || o->mass % 2 || (o->tbstart >= 0 && o->tbstart % 2)) {
         fprintf(stderr, "islbm_run: steps and the cadences must be even (two steps per iteration)\n");
         return -1;
     }
-    if (o->dropin && (o->p.precision != ISLBM_FP64 || o->init != 0 || o->tbstart >= 0 || o->mass > 0 || o->amp != 0.0)) {
+    if (o->dropin && (o->p.precision != ISLBM_FP64 || o->init != 0 || o->step0 != 0 || o->tbstart >= 0 || o->mass > 0 || o->amp != 0.0)) {
         fprintf(stderr, "islbm_run: --dropin runs the reference's FP64 calls from the equilibrium start only\n");
         return -1;
     }
@@ -211,6 +217,8 @@ static int run_native(const options* o)
     } else if (islbm_checkpoint_read(s, o->dir, 0)) {
         rc = die("islbm_checkpoint_read", islbm_io_last_error());
     }
+    /* the step count a restart resumes from (the result files carry none) */
+    if (!rc && o->step0 > 0 && islbm_set_step_count(s, o->step0)) rc = die("islbm_set_step_count", islbm_last_error());
     if (!rc && islbm_sync(s)) rc = die("islbm_sync", islbm_last_error());
     t0 = now_s();
     for (int step = 0; !rc && step < o->steps; step += 2) {

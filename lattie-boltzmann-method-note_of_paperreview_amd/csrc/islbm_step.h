@@ -51,7 +51,35 @@ struct Codec<Store::BF16> {
         // round to nearest even (v_cvt_pk_bf16_f32)
         p[i] = __builtin_bit_cast(uint16_t, (__bf16)(v * sc));
     }
+    // Stochastic rounding (v_cvt_sr_bf16_f32): the magnitude rounds up when its 16 discarded
+    // mantissa bits plus the high 16 bits of rnd carry into bit 16, so it rounds up with
+    // probability (discarded bits) / 2^16 over uniform rnd, and the stored code is an
+    // unbiased estimate of v * sc.  With round to nearest, the rounding error of a smooth
+    // field is the same at neighbouring nodes and step after step: it acts as a body force
+    // and slows the bulk flow (12% in 4,000 steps at 64 x 128 x 64, Re 10595).
+    __device__ static raw sr(C v, float sc, unsigned rnd)
+    {
+        typedef __attribute__((ext_vector_type(2))) __bf16 bf2;
+        const bf2 r = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf2, 0u), v * sc, rnd, false);
+        return __builtin_bit_cast(uint16_t, r[0]);
+    }
 };
+
+// Dither of the stochastic rounding of BF16 stores: a hash of the unique node index and the
+// solve's step count (the lowbias32 integer finaliser), then direction q takes
+// hash + q * 0x9E3779B9.  It depends on nothing else, so a solve's stores are the same in
+// either kernel form, tile width, alone or grouped, eager or graph-replayed.
+__device__ inline unsigned sr_hash(unsigned node, unsigned step)
+{
+    unsigned x = node * 0x9E3779B1u + step * 0x85EBCA77u;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+__host__ __device__ constexpr unsigned sr_lane(int q) { return (unsigned)q * 0x9E3779B9u; }
 
 template <>
 struct Codec<Store::FP16> {
@@ -89,6 +117,7 @@ struct StepArgs {
     const double* flux_dz;
     double* flux_acc;
     int flux_plane, flux_par, flux_nslot;
+    double* step_ctr;  // BF16: step counters by buffer parity (StepLaunch::step_ctr), else null
 };
 
 template <Store ST>
@@ -127,6 +156,7 @@ inline StepArgs<ST> make_step_args(const StepLaunch& L)
     a.flux_plane = L.flux_plane;
     a.flux_par = L.flux_par;
     a.flux_nslot = L.flux_nslot;
+    a.step_ctr = L.step_ctr;
     return a;
 }
 

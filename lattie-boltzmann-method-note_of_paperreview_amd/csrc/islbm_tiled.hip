@@ -200,6 +200,12 @@ struct BufIO {
             return CD::ld(&v, 0, isc);
         }
     }
+    // BF16 with stochastic rounding (Codec<BF16>::sr)
+    __device__ static void st_sr(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float x, float sc, unsigned rnd)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, CD::sr(x, sc, rnd)), r, (int)voff, (int)soff,
+                                              kStoreAux);
+    }
     __device__ static void st(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float x, float sc)
     {
         raw v;
@@ -1003,6 +1009,14 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             for (int e = 0; e < a.flux_nslot; ++e) t += prev[e];
             a.flux_acc[0] += t;
         }
+        // BF16: the step count that dithers the stochastic rounding of the stores (slot
+        // flux_par, written by the previous launch); one lane of the launch writes the next
+        unsigned sr_step = 0;
+        if constexpr (ST == Store::BF16) {
+            const double ctr = ((cdouble*)a.step_ctr)[a.flux_par];
+            sr_step = (unsigned)(unsigned long long)ctr;
+            if (wk == 0 && cw == 0 && lane == 0) a.step_ctr[1 - a.flux_par] = ctr + 1.0;
+        }
         // walls, collision and stores of target level tk.  Lanes outside the grid (partial
         // tiles) hold the wrapped column or row, whose values they compute bit for bit like
         // its owner, so their stores duplicate the owner's: the stores need no lane mask,
@@ -1046,8 +1060,14 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             if (probe) psum += ((double)uy * pdx) * ((const __attribute__((address_space(4))) double*)a.flux_dz)[tk];
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
+            if constexpr (ST == Store::BF16) {
+                const unsigned h = sr_hash((unsigned)node, sr_step);
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) IO::st(dst(q), own_b, qoff(q) + lv, F[q], a.scale[q]);
+                for (int q = 0; q < NQ; ++q) IO::st_sr(dst(q), own_b, qoff(q) + lv, F[q], a.scale[q], h + sr_lane(q));
+            } else {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) IO::st(dst(q), own_b, qoff(q) + lv, F[q], a.scale[q]);
+            }
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), macdst(0), (int)own4, (int)(lf + macoff(0)), kStoreAux);
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), macdst(1), (int)own4, (int)(lf + macoff(1)), kStoreAux);
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(uy), macdst(2), (int)own4, (int)(lf + macoff(2)), kStoreAux);

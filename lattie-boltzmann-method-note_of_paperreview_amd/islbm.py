@@ -25,7 +25,7 @@ HEADER_SYMBOLS = [
     "islbm_default_params", "islbm_required_bytes", "islbm_create", "islbm_destroy",
     "islbm_init_equilibrium", "islbm_upload_f", "islbm_download_f", "islbm_download_macros",
     "islbm_step", "islbm_sync", "islbm_modify_forcing", "islbm_get_force", "islbm_set_force",
-    "islbm_set_rho_modify", "islbm_get_flux_sum", "islbm_scalars", "islbm_step_count",
+    "islbm_set_rho_modify", "islbm_get_flux_sum", "islbm_scalars", "islbm_step_count", "islbm_set_step_count",
     "islbm_kernel_in_use", "islbm_tile_shape", "islbm_get_table", "islbm_time_steps", "islbm_last_error",
     "islbm_selftest_collision", "islbm_copy_bandwidth",
     "islbm_ref_init", "islbm_ref_shutdown", "islbm_ref_device_arrays",
@@ -96,6 +96,7 @@ def lib():
     L.islbm_get_flux_sum.argtypes = [vp, dp]
     L.islbm_scalars.argtypes = [vp, dp]
     L.islbm_step_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.islbm_set_step_count.argtypes = [vp, ctypes.c_int64]
     L.islbm_kernel_in_use.argtypes = [vp, ctypes.POINTER(i32)]
     L.islbm_tile_shape.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
     L.islbm_get_table.argtypes = [vp, ctypes.c_char_p, dp, i32]
@@ -279,6 +280,9 @@ class Solver:
         v = ctypes.c_int64()
         _check(lib().islbm_step_count(self.h, ctypes.byref(v)), "islbm_step_count")
         return v.value
+
+    def set_step_count(self, steps: int):
+        _check(lib().islbm_set_step_count(self.h, int(steps)), "islbm_set_step_count")
 
     def kernel_in_use(self) -> int:
         v = ctypes.c_int32()

@@ -114,7 +114,7 @@ def test_c_driver_rejects_bad_arguments_before_the_gpu(pkg):
     exe = os.path.join(os.path.dirname(pkg.islbm.LIB_PATH), "islbm_run")
     assert os.access(exe, os.X_OK), exe
     for args in (["--steps", "3"], ["--ndtfrc", "7"], ["--ndtmit", "5"], ["--mass", "3"], ["--tbstart", "1"],
-                 ["--dropin", "--precision", "fp32"], ["--dropin", "--init", "1"], ["--dropin", "--amp", "1e-3"],
+                 ["--dropin", "--precision", "fp32"], ["--dropin", "--init", "1"], ["--step0", "-2"], ["--dropin", "--step0", "4"], ["--dropin", "--amp", "1e-3"],
                  ["--grid", "8,8"], ["--precision", "fp8"], ["--bogus", "1"], ["--steps"]):
         r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=30)
         assert r.returncode == 2, (args, r.returncode, r.stderr)

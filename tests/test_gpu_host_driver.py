@@ -114,16 +114,19 @@ def test_native_fp32_matches_the_oracle(pkg, po, tmp_path):
     assert pkg.rel_l2_vec([unique(result(tmp_path, n)) for n in ("u", "v", "w")], (u, v, w)) <= 2e-5
 
 
-def test_restart_continues_the_run(pkg, tmp_path):
+@pytest.mark.parametrize("prec", ["fp64", "bf16"])
+def test_restart_continues_the_run(pkg, tmp_path, prec):
     """INIT = 1 (InitialUsingBkpData) from the driver's own result files after 10 steps, then
-    10 more steps, equals 20 uninterrupted steps; statistics from step 10 on count 10 samples."""
+    10 more steps from step count 10, equals 20 uninterrupted steps (BF16 included: its
+    stores are dithered by the step count); statistics from step 10 on count 10 samples."""
     full, part = tmp_path / "full", tmp_path / "part"
-    run(pkg, full, "--steps", "20", "--ndtfrc", "10", "--tbstart", "10")
-    run(pkg, part, "--steps", "10", "--ndtfrc", "10")
-    run(pkg, part, "--steps", "10", "--ndtfrc", "10", "--init", "1", "--tbstart", "0")
+    run(pkg, full, "--steps", "20", "--ndtfrc", "10", "--tbstart", "10", "--precision", prec)
+    run(pkg, part, "--steps", "10", "--ndtfrc", "10", "--precision", prec)
+    run(pkg, part, "--steps", "10", "--ndtfrc", "10", "--init", "1", "--step0", "10", "--tbstart", "0", "--precision", prec)
     for name in NAMES:
         assert np.max(np.abs(result(full, name) - result(part, name))) < 1e-13, name
-    assert abs(force(full) - force(part)) <= 1e-13 * abs(force(full))
+    # 0_force.dat holds 15 decimals ("%.15lf"): the restart starts from the force rounded there
+    assert abs(force(full) - force(part)) <= 2e-15
     for d in (full, part):
         with open(os.path.join(d, "statistics", "accu.dat")) as fp:
             assert int(fp.read()) == 10

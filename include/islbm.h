@@ -39,7 +39,8 @@ extern "C" {
 #define ISLBM_FP32 1 /* FP32 storage and arithmetic                                    */
 #define ISLBM_BF16 2 /* BF16 storage of (f_i - w_i) * s_i, FP32 arithmetic; the step   */
                      /* rounds its stores stochastically (dither: node, step count)   */
-#define ISLBM_FP16 3 /* FP16 storage of (f_i - w_i) * s_i, FP32 arithmetic              */
+#define ISLBM_FP16 3 /* FP16 storage of (f_i - w_i) * s_i, FP32 arithmetic; stores as  */
+                     /* BF16's: stochastic rounding                                   */
 
 /* wall-normal mesh */
 #define ISLBM_MESH_BISECT 0  /* tanh parameter a bisected so that the first cell == minSize
@@ -206,7 +207,7 @@ int islbm_scalars(islbm_solver* s, double out[8]);
 /* Steps taken since creation / the last init. */
 int islbm_step_count(islbm_solver* s, int64_t* steps);
 /* Sets the step count (for a restart: the reference's checkpoint files carry none).  The
- * count places the forcing-controller updates and, with BF16 storage, dithers the
+ * count places the forcing-controller updates and, with 16-bit storage, dithers the
  * stochastic rounding of the step's stores, so a restart given the count it stopped at
  * continues bit for bit.  No counterpart in the reference (its main keeps the count). */
 int islbm_set_step_count(islbm_solver* s, int64_t steps);

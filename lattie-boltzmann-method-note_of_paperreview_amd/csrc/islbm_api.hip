@@ -144,7 +144,7 @@ struct islbm_solver {
     void *bq = nullptr, *bw = nullptr, *les = nullptr;
     int* bflag = nullptr;
     double* fluxw = nullptr;
-    double* scal = nullptr;  // [0] force, [1] rho_modify, [2] flux accumulator, [3..4] BF16 step counters
+    double* scal = nullptr;  // [0] force, [1] rho_modify, [2] flux accumulator, [3..4] 16-bit step counters
     DevTables<double> td{};
     DevTables<float> tf{};
     float scale[NQ];
@@ -358,7 +358,7 @@ StepLaunch make_launch(islbm_solver* s, int cur)
     L.flux_par = cur;
     L.flux_nslot = s->flux_nslot;
     L.wide_addr = s->wide_addr;
-    L.step_ctr = s->store == Store::BF16 ? s->scal + 3 : nullptr;
+    L.step_ctr = (s->store == Store::BF16 || s->store == Store::FP16) ? s->scal + 3 : nullptr;
     return L;
 }
 
@@ -538,7 +538,7 @@ int islbm_init_equilibrium(islbm_solver* s, double rho0, const double u0[3], dou
     s->cur = 0;
     HIPCHK(launch_init_equilibrium(s->stream, s->store, s->f[0], s->qs, s->mac[0], s->n, s->p.nx, s->p.ny, s->p.nz,
                                    rho0, u0, amp, seed, s->scale));
-    // force, rho_modify, flux accumulator, and the step counters of BF16 storage
+    // force, rho_modify, flux accumulator, and the step counters of 16-bit storage
     const double sc[5] = {s->g.force0, 0.0, 0.0, 0.0, 0.0};
     HIPCHK(hipMemcpyAsync(s->scal, sc, sizeof sc, hipMemcpyHostToDevice, s->stream));
     if (int rc = flux_refresh(s)) return rc;
@@ -925,7 +925,7 @@ int islbm_set_step_count(islbm_solver* s, int64_t steps)
     if (!s || steps < 0) return fail(ISLBM_ERR_ARG, "bad argument");
     HIPCHK(hipSetDevice(s->p.device));
     HIPCHK(hipStreamSynchronize(s->stream));
-    // the device copy read by the BF16 step, in both parity slots
+    // the device copy read by the 16-bit step, in both parity slots
     const double c[2] = {(double)steps, (double)steps};
     HIPCHK(hipMemcpy(s->scal + 3, c, sizeof c, hipMemcpyHostToDevice));
     s->steps = steps;

@@ -98,7 +98,7 @@ struct StepLaunch {
     int flux_par;         // parity of this step: writes flux_part[par], adds flux_part[1 - par]
     int flux_nslot;
     bool wide_addr;       // tiled kernel: one resource per population array (forced, or for grids that need it)
-    // BF16 storage: the solve's step counter in two slots by buffer parity (flux_par).  A
+    // 16-bit storage: the solve's step counter in two slots by buffer parity (flux_par).  A
     // launch reads slot flux_par, and one thread of it writes slot 1 - flux_par = that + 1, so
     // eager, graph-replayed and grouped steps all see the solve's own step count.  Null for
     // the other storage formats.

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_collide_pernode(StepArgs<ST> a)
     collide_pairs<CD::kDev>(F, (C)*a.rho_modify, (C)a.dt, (C)(*a.force), a.les ? a.t.les_delta[(long long)k * ny + j] : C(0),
                            a.les, (C)a.tau, (C)a.cs, rho, ux, uy, uz, tau_eff, (C)(2.0 * a.tau - 1.0));
 
-    if constexpr (ST == Store::BF16) {
+    if constexpr (ST == Store::BF16 || ST == Store::FP16) {
         // stochastic rounding dithered by (node, step); thread 0 advances the step counter
         const double ctr = a.step_ctr[a.flux_par];
         const unsigned h = sr_hash((unsigned)node, (unsigned)(unsigned long long)ctr);

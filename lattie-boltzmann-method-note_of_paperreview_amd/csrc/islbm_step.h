@@ -65,7 +65,7 @@ struct Codec<Store::BF16> {
     }
 };
 
-// Dither of the stochastic rounding of BF16 stores: a hash of the unique node index and the
+// Dither of the stochastic rounding of 16-bit stores (BF16 and FP16): a hash of the unique node index and the
 // solve's step count (the lowbias32 integer finaliser), then direction q takes
 // hash + q * 0x9E3779B9.  It depends on nothing else, so a solve's stores are the same in
 // either kernel form, tile width, alone or grouped, eager or graph-replayed.
@@ -88,6 +88,13 @@ struct Codec<Store::FP16> {
     static constexpr bool kDev = true;
     __device__ static C ld(const raw* p, long long i, float isc) { return (float)p[i] * isc; }
     __device__ static void st(raw* p, long long i, C v, float sc) { p[i] = (_Float16)(v * sc); }
+    // stochastic rounding (v_cvt_sr_f16_f32), as Codec<BF16>::sr
+    __device__ static raw sr(C v, float sc, unsigned rnd)
+    {
+        typedef __attribute__((ext_vector_type(2))) _Float16 h2;
+        const h2 r = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, 0u), v * sc, rnd, false);
+        return r[0];
+    }
 };
 
 template <Store ST>
@@ -117,7 +124,7 @@ struct StepArgs {
     const double* flux_dz;
     double* flux_acc;
     int flux_plane, flux_par, flux_nslot;
-    double* step_ctr;  // BF16: step counters by buffer parity (StepLaunch::step_ctr), else null
+    double* step_ctr;  // 16-bit storage: step counters by buffer parity (StepLaunch::step_ctr), else null
 };
 
 template <Store ST>

@@ -200,7 +200,7 @@ struct BufIO {
             return CD::ld(&v, 0, isc);
         }
     }
-    // BF16 with stochastic rounding (Codec<BF16>::sr)
+    // 16-bit storage with stochastic rounding (Codec<BF16>::sr, Codec<FP16>::sr)
     __device__ static void st_sr(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float x, float sc, unsigned rnd)
     {
         __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, CD::sr(x, sc, rnd)), r, (int)voff, (int)soff,
@@ -1009,10 +1009,10 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             for (int e = 0; e < a.flux_nslot; ++e) t += prev[e];
             a.flux_acc[0] += t;
         }
-        // BF16: the step count that dithers the stochastic rounding of the stores (slot
-        // flux_par, written by the previous launch); one lane of the launch writes the next
+        // 16-bit storage: the step count that dithers the stochastic rounding of the stores
+        // (slot flux_par, written by the previous launch); one lane of the launch writes the next
         unsigned sr_step = 0;
-        if constexpr (ST == Store::BF16) {
+        if constexpr (k16) {
             const double ctr = ((cdouble*)a.step_ctr)[a.flux_par];
             sr_step = (unsigned)(unsigned long long)ctr;
             if (wk == 0 && cw == 0 && lane == 0) a.step_ctr[1 - a.flux_par] = ctr + 1.0;
@@ -1060,7 +1060,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             if (probe) psum += ((double)uy * pdx) * ((const __attribute__((address_space(4))) double*)a.flux_dz)[tk];
             const unsigned lv = SZ * (unsigned)tk * (unsigned)plane;
             const unsigned lf = 4u * (unsigned)tk * (unsigned)plane;
-            if constexpr (ST == Store::BF16) {
+            if constexpr (k16) {
                 const unsigned h = sr_hash((unsigned)node, sr_step);
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) IO::st_sr(dst(q), own_b, qoff(q) + lv, F[q], a.scale[q], h + sr_lane(q));

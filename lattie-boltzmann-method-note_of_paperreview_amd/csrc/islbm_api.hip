@@ -55,7 +55,7 @@ Store store_of(int precision)
 }
 
 // 16-bit deviation scale per direction: (f - w_q) / w_q is O(Ma) for every q.
-float dev_scale(int precision, int q) { return precision >= ISLBM_BF16 ? (float)(1.0 / wq(q)) : 1.0f; }
+float dev_scale(int precision, int q) { return precision >= ISLBM_BF16 ? store_scale16(q) : 1.0f; }
 
 uint16_t f32_to_bf16_rne(float v)
 {

@@ -18,6 +18,11 @@ namespace islbm {
 template <Store ST>
 struct Codec;
 
+// Per-direction scale of the 16-bit storage forms, s_q = 1 / w_q: 3, 18 and 36, exact in
+// FP32.  Host and kernels take these constants (kernels as literals, not as arguments held
+// in scalar registers).
+__host__ __device__ constexpr float store_scale16(int q) { return q == 0 ? 3.0f : q <= 6 ? 18.0f : 36.0f; }
+
 template <>
 struct Codec<Store::F64> {
     using raw = double;
@@ -407,7 +412,11 @@ template <Store ST, class C>
 __device__ inline void apply_walls(C (&F)[NQ], const StepArgs<ST>& a, long long node, int i, int j, int k,
                                    const int* xo, const int* yo, long long plane)
 {
-    auto pop = [&](int q) { return PopReader<ST>{a.fo + (long long)q * a.qs, a.inv_scale[q]}; };
+    // 16-bit codes: the inverse scale as a literal (store_scale16); 1 for the other formats
+    constexpr bool k16 = ST == Store::BF16 || ST == Store::FP16;
+    auto pop = [&](int q) {
+        return PopReader<ST>{a.fo + (long long)q * a.qs, k16 ? 1.0f / store_scale16(q) : 1.0f};
+    };
     apply_walls_with<ST>(F, a, pop, node, i, j, k, xo, yo, plane);
 }
 

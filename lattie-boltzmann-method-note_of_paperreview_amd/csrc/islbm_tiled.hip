@@ -380,6 +380,9 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     constexpr bool DEC = LY::DEC;
     constexpr int TY = LY::TY, NRW = LY::NRW, HY = LY::HY, Y_SZ = LY::Y_SZ, HO_SZ = LY::HO_SZ, NW16 = LY::NW16;
     __shared__ __attribute__((aligned(16))) float lds[LY::LDS_FLOATS];
+    // per-direction scale of 16-bit codes and its inverse, as literals (1 for FP32 storage)
+    auto scl_of = [](int q) { return k16 ? store_scale16(q) : 1.0f; };
+    auto isc_of = [](int q) { return k16 ? 1.0f / store_scale16(q) : 1.0f; };
 
     // XCD-aware work mapping: block b runs on XCD b % 8
     const int per_xcd = tg.nblocks / 8;
@@ -428,7 +431,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
     // directions 0, 1, 2 of a target (no z dependence) from its staged rows P: the node's own
     // 0 and the 7 x taps of 1 and 2 (weights w1, w2), in FP32 deviation form
     auto reduce012 = [&](const float* P, const float* w1, const float* w2, float& f0, float& f1, float& f2) {
-        const float isc0 = a.inv_scale[0], isc1 = a.inv_scale[1], isc2 = a.inv_scale[2];
+        const float isc0 = isc_of(0), isc1 = isc_of(1), isc2 = isc_of(2);
         if constexpr (W && k16) {
             // 16-bit codes: direction 0 at the nodes, 1 and 2 with the x halo
             const unsigned short* P16 = (const unsigned short*)P;
@@ -613,14 +616,14 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
         auto decode = [&](int b) {
             if constexpr (k16 && !W) {
                 float* L = lds + b * STAGE;
-                decode_region(L + OFF_XY + w * XY_SZ, XY_SZ, a.inv_scale[7 + w]);
-                decode_region(L + OFF_X + w * X_SZ, X_SZ, a.inv_scale[11 + w]);
+                decode_region(L + OFF_XY + w * XY_SZ, XY_SZ, isc_of(7 + w));
+                decode_region(L + OFF_X + w * X_SZ, X_SZ, isc_of(11 + w));
 #pragma unroll
                 for (int d = 0; d < 6; ++d)
-                    decode_region(L + OFF_Y + d * Y_SZ + 2 * w * TX, 2 * TX, a.inv_scale[qy(d)]);
+                    decode_region(L + OFF_Y + d * Y_SZ + 2 * w * TX, 2 * TX, isc_of(qy(d)));
                 if (w >= TY) {
-                    decode_region(L + OFF_Z + (w - TY) * TX, TX, a.inv_scale[5]);
-                    decode_region(L + OFF_Z + (TY + w - TY) * TX, TX, a.inv_scale[6]);
+                    decode_region(L + OFF_Z + (w - TY) * TX, TX, isc_of(5));
+                    decode_region(L + OFF_Z + (TY + w - TY) * TX, TX, isc_of(6));
                 }
             }
         };
@@ -804,7 +807,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                     };
                     auto scl = [&](float v, int q) -> float {
                         if constexpr (code16)
-                            return v * a.inv_scale[q];
+                            return v * isc_of(q);
                         else
                             return v;
                     };
@@ -1035,7 +1038,7 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                 // 0.5-0.9 us slower that way and keep the global-pointer reads
                 if constexpr (ST == Store::BF16) {
                     auto pop = [&](int q) {
-                        return [&, q](long long n) { return IO::ld(src(q), SZ * (unsigned)n, qoff(q), a.inv_scale[q]); };
+                        return [&, q](long long n) { return IO::ld(src(q), SZ * (unsigned)n, qoff(q), isc_of(q)); };
                     };
                     apply_walls_with<ST>(F, a, pop, node, iw, jw, tk, xo, yo, plane);
                 } else {
@@ -1063,10 +1066,10 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
             if constexpr (k16) {
                 const unsigned h = sr_hash((unsigned)node, sr_step);
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) IO::st_sr(dst(q), own_b, qoff(q) + lv, F[q], a.scale[q], h + sr_lane(q));
+                for (int q = 0; q < NQ; ++q) IO::st_sr(dst(q), own_b, qoff(q) + lv, F[q], scl_of(q), h + sr_lane(q));
             } else {
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) IO::st(dst(q), own_b, qoff(q) + lv, F[q], a.scale[q]);
+                for (int q = 0; q < NQ; ++q) IO::st(dst(q), own_b, qoff(q) + lv, F[q], scl_of(q));
             }
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rho), macdst(0), (int)own4, (int)(lf + macoff(0)), kStoreAux);
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ux), macdst(1), (int)own4, (int)(lf + macoff(1)), kStoreAux);

@@ -33,6 +33,12 @@ __host__ __device__ constexpr int cz(int q)
 }
 // weights: initialization.h:10-12
 __host__ __device__ constexpr double wq(int q) { return q == 0 ? 1.0 / 3.0 : (q <= 6 ? 1.0 / 18.0 : 1.0 / 36.0); }
+// Per-direction scale of the 16-bit storage forms, s_q = 1 / w_q: 3, 18 and 36, exact in
+// FP32.  Host and kernels take these constants (kernels as literals, not as arguments held
+// in scalar registers).
+__host__ __device__ constexpr float store_scale16(int q) { return q == 0 ? 3.0f : q <= 6 ? 18.0f : 36.0f; }
+static_assert(store_scale16(0) == 1.0 / wq(0) && store_scale16(3) == 1.0 / wq(3) && store_scale16(9) == 1.0 / wq(9),
+              "16-bit scales are 1 / w_q");
 
 // D3Q19 d'Humieres moment basis row a evaluated at velocity q (MRT_Matrix.h:27-46); the
 // rows are orthogonal, so M^-1 = M^T diag(|row|^2)^-1 (the MRT_Matrix.h:49-68 table).

@@ -18,11 +18,6 @@ namespace islbm {
 template <Store ST>
 struct Codec;
 
-// Per-direction scale of the 16-bit storage forms, s_q = 1 / w_q: 3, 18 and 36, exact in
-// FP32.  Host and kernels take these constants (kernels as literals, not as arguments held
-// in scalar registers).
-__host__ __device__ constexpr float store_scale16(int q) { return q == 0 ? 3.0f : q <= 6 ? 18.0f : 36.0f; }
-
 template <>
 struct Codec<Store::F64> {
     using raw = double;

@@ -1033,10 +1033,11 @@ __device__ __attribute__((always_inline)) inline void tiled_body(const StepArgs<
                     xo[l] = wrap_index(iw - 3 + l, nx);
                     yo[l] = wrap_index(jw - 3 + l, ny) * nx;
                 }
-                // BF16: wall reads through the buffer resource (no 64-bit address per array held in
-                // scalar registers: BF16 152.9 -> 148.9 us at 128x256x128); FP32 and FP16 measured
-                // 0.5-0.9 us slower that way and keep the global-pointer reads
-                if constexpr (ST == Store::BF16) {
+                // 16-bit storage: wall reads through the buffer resource (no 64-bit address per array
+                // held in scalar registers: BF16 152.9 -> 148.9 us, and with the literal scales FP16
+                // 140.8 -> 139.9 us at 128x256x128); FP32 times the same either way and keeps the
+                // global-pointer reads
+                if constexpr (k16) {
                     auto pop = [&](int q) {
                         return [&, q](long long n) { return IO::ld(src(q), SZ * (unsigned)n, qoff(q), isc_of(q)); };
                     };
